@@ -1,0 +1,1020 @@
+// The C ABI of libgargantua_hip.so (include/gargantua.h): objects, memory
+// budget, argument checks and kernel launches. Every entry point returns a
+// status and leaves one sentence in garg_last_error() on failure.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gargantua.h"
+#include "../../include/gargantua_constants.h"
+#include "host_tables.hpp"
+#include "launch.hpp"
+
+using namespace garg;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr, what)                                                                 \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) return fail(GARG_E_DEVICE, "%s: %s", what, hipGetErrorString(e_)); \
+    } while (0)
+
+#define REQUIRE(cond, ...)                                             \
+    do {                                                               \
+        if (!(cond)) return fail(GARG_E_ARGUMENT, __VA_ARGS__);        \
+    } while (0)
+
+double max_impact_parameter(double r) { return r / std::sqrt(1.0 - R_SCHWARZSCHILD / r); }
+
+}  // namespace
+
+struct garg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t allocated = 0;
+    garg_device_info info{};
+    float4* blackbody = nullptr;
+    float* inverse_cdf = nullptr;
+    double temperature_scale = 0.0;
+    // Scratch of the batched march and the probes.
+    BatchRequest* batch_requests = nullptr;
+    BatchRequest* batch_staging = nullptr;  // pinned
+    hipEvent_t batch_staging_free = nullptr;
+    unsigned long long* queue = nullptr;
+    garg_march_counters* batch_totals = nullptr;
+    garg_march_counters* scratch_counters = nullptr;
+    void* probe_scratch = nullptr;
+    uint64_t probe_scratch_bytes = 0;
+};
+
+struct garg_stream {
+    garg_ctx* ctx;
+    hipStream_t s;
+};
+
+struct garg_event {
+    garg_ctx* ctx;
+    hipEvent_t e;
+};
+
+struct garg_particles {
+    garg_ctx* ctx;
+    uint32_t count;
+    uint64_t seed;
+    uint32_t potential;
+    void* base;
+    uint64_t bytes;
+    ParticleView view;
+};
+
+struct garg_volume {
+    garg_ctx* ctx;
+    uint32_t size;
+    float half_xy, half_z;
+    void* base;
+    uint64_t bytes;
+    uint32_t* emission;
+    int32_t* momentum;
+    uint32_t* mass;
+    Voxel* voxels;
+};
+
+struct garg_tables {
+    garg_ctx* ctx;
+    double sphere_radius, entry_radius, camera_radius;
+    float* data;  // sphere, camera, sky
+    float sphere_max_b, sphere_table_max_b, camera_max_b, sky_min_b, sky_max_b;
+};
+
+struct garg_targets {
+    garg_ctx* ctx;
+    uint32_t width, height;
+    void* base;
+    uint64_t bytes;
+    h4* color;
+    h4* debug;
+    float* depth;
+    garg_march_counters* counters;
+    uint8_t* rgba8;
+};
+
+struct garg_bake {
+    garg_ctx* ctx;
+    uint32_t width, height, capacity;
+    void* base;
+    uint64_t bytes;
+    h4* samples;
+    uint4* headers;
+    uint32_t* overflow;
+};
+
+namespace {
+
+hipStream_t stream_of(garg_ctx* ctx, garg_stream* s) { return s ? s->s : ctx->stream; }
+
+uint64_t align256(uint64_t b) { return (b + 255u) & ~(uint64_t)255u; }
+
+int ctx_alloc(garg_ctx* ctx, uint64_t bytes, void** out, const char* what) {
+    bytes = align256(std::max<uint64_t>(bytes, 256));
+    if (ctx->allocated + bytes > ctx->budget)
+        return fail(GARG_E_BUDGET, "%s needs %.1f MB but the context budget has %.1f MB of %.1f MB left.", what,
+                    bytes / 1e6, (ctx->budget - ctx->allocated) / 1e6, ctx->budget / 1e6);
+    HIP_TRY(hipMalloc(out, bytes), what);
+    ctx->allocated += bytes;
+    return GARG_OK;
+}
+
+void ctx_free(garg_ctx* ctx, void* p, uint64_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    ctx->allocated -= align256(std::max<uint64_t>(bytes, 256));
+}
+
+int sync_stream(hipStream_t s, const char* what) {
+    HIP_TRY(hipStreamSynchronize(s), what);
+    return GARG_OK;
+}
+
+Scene make_scene(garg_ctx* ctx, garg_volume* volume, garg_tables* tables) {
+    Scene sc{};
+    if (volume) sc.volume = VolumeView{volume->voxels, (int)volume->size, volume->half_xy, volume->half_z};
+    if (tables) {
+        sc.sphere_table = tables->data;
+        sc.camera_table = tables->data + SWEEP_TABLE_SIZE;
+        sc.sky_table = tables->data + 2 * SWEEP_TABLE_SIZE;
+    }
+    sc.blackbody = ctx->blackbody;
+    return sc;
+}
+
+int check_uniforms(const garg_march_uniforms* u) {
+    REQUIRE(u, "uniforms are required.");
+    REQUIRE(u->resolution[0] > 0 && u->resolution[1] > 0, "the resolution must be positive.");
+    REQUIRE(u->spin >= 0.0f && u->spin < 1.0f, "spin must be in [0, 1), not %g.", (double)u->spin);
+    REQUIRE(u->geodesic.step_cap <= (1u << 24), "the step cap %u is above 2^24.", u->geodesic.step_cap);
+    REQUIRE(u->geodesic.escape_radius > u->geodesic.capture_radius, "the escape radius must exceed the capture radius.");
+    float r = std::sqrt(u->camera_position[0] * u->camera_position[0] + u->camera_position[1] * u->camera_position[1] +
+                        u->camera_position[2] * u->camera_position[2]);
+    REQUIRE(std::isfinite(r) && r > 2.5f, "the camera must sit outside r = 2.5, not at r = %g.", (double)r);
+    return GARG_OK;
+}
+
+// Uniforms as the device reads them: a zero z extent means a cube.
+garg_march_uniforms device_uniforms(const garg_march_uniforms* u, garg_volume* volume) {
+    garg_march_uniforms d = *u;
+    if (volume) {
+        d.volume_half_extent = volume->half_xy;
+        d.volume_half_extent_z = volume->half_z;
+    } else if (d.volume_half_extent_z <= 0.0f) {
+        d.volume_half_extent_z = d.volume_half_extent;
+    }
+    return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* garg_last_error(void) { return g_error.c_str(); }
+const char* garg_version(void) { return "gargantua-mi355x 0.1.0 (gfx950)"; }
+
+int garg_ctx_create(int device, uint64_t budget_bytes, garg_ctx** out) {
+    REQUIRE(out, "out is required.");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count), "counting HIP devices");
+    REQUIRE(count > 0, "no HIP device is visible.");
+    REQUIRE(device >= 0 && device < count, "device %d does not exist; %d are visible.", device, count);
+    HIP_TRY(hipSetDevice(device), "selecting the device");
+    garg_ctx* ctx = new garg_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device), "reading device properties");
+    snprintf(ctx->info.name, sizeof(ctx->info.name), "%s", prop.name);
+    snprintf(ctx->info.arch, sizeof(ctx->info.arch), "%s", prop.gcnArchName);
+    ctx->info.compute_units = (uint32_t)prop.multiProcessorCount;
+    ctx->info.clock_khz = (uint32_t)prop.clockRate;
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b), "reading device memory");
+    ctx->info.total_bytes = total_b;
+    ctx->info.free_bytes = free_b;
+    ctx->budget = budget_bytes ? budget_bytes : (uint64_t)(0.9 * (double)free_b);
+    ctx->info.budget_bytes = ctx->budget;
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "creating the default stream");
+
+    std::vector<float> bb, cdf;
+    build_blackbody_table(bb);
+    build_inverse_cdf(cdf);
+    ctx->temperature_scale = temperature_scale();
+    int rc;
+    if ((rc = ctx_alloc(ctx, bb.size() * 4, (void**)&ctx->blackbody, "the blackbody table"))) return rc;
+    if ((rc = ctx_alloc(ctx, cdf.size() * 4, (void**)&ctx->inverse_cdf, "the seeding table"))) return rc;
+    HIP_TRY(hipMemcpy(ctx->blackbody, bb.data(), bb.size() * 4, hipMemcpyHostToDevice), "uploading the blackbody table");
+    HIP_TRY(hipMemcpy(ctx->inverse_cdf, cdf.data(), cdf.size() * 4, hipMemcpyHostToDevice), "uploading the seeding table");
+    if ((rc = ctx_alloc(ctx, sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS, (void**)&ctx->batch_requests,
+                        "the batch request table")))
+        return rc;
+    HIP_TRY(hipHostMalloc((void**)&ctx->batch_staging, sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS, 0),
+            "allocating pinned staging");
+    HIP_TRY(hipEventCreateWithFlags(&ctx->batch_staging_free, hipEventDisableTiming), "creating an event");
+    if ((rc = ctx_alloc(ctx, 8 * 16 * sizeof(unsigned long long), (void**)&ctx->queue, "the ray queue"))) return rc;
+    if ((rc = ctx_alloc(ctx, sizeof(garg_march_counters), (void**)&ctx->batch_totals, "batch counters"))) return rc;
+    if ((rc = ctx_alloc(ctx, sizeof(garg_march_counters), (void**)&ctx->scratch_counters, "counters"))) return rc;
+    *out = ctx;
+    return GARG_OK;
+}
+
+int garg_ctx_destroy(garg_ctx* ctx) {
+    if (!ctx) return GARG_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx_free(ctx, ctx->blackbody, BLACKBODY_TABLE_SIZE * 16);
+    ctx_free(ctx, ctx->inverse_cdf, INVERSE_CDF_COUNT * 4);
+    ctx_free(ctx, ctx->batch_requests, sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS);
+    ctx_free(ctx, ctx->queue, 8 * 16 * sizeof(unsigned long long));
+    ctx_free(ctx, ctx->batch_totals, sizeof(garg_march_counters));
+    ctx_free(ctx, ctx->scratch_counters, sizeof(garg_march_counters));
+    ctx_free(ctx, ctx->probe_scratch, ctx->probe_scratch_bytes);
+    if (ctx->batch_staging) (void)hipHostFree(ctx->batch_staging);
+    if (ctx->batch_staging_free) (void)hipEventDestroy(ctx->batch_staging_free);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GARG_OK;
+}
+
+int garg_device_info_get(garg_ctx* ctx, garg_device_info* out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b), "reading device memory");
+    ctx->info.free_bytes = free_b;
+    *out = ctx->info;
+    return GARG_OK;
+}
+
+int garg_ctx_allocated(garg_ctx* ctx, uint64_t* bytes) {
+    REQUIRE(ctx && bytes, "ctx and bytes are required.");
+    *bytes = ctx->allocated;
+    return GARG_OK;
+}
+
+int garg_stream_create(garg_ctx* ctx, garg_stream** out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    hipStream_t s;
+    // Default priority: every stream of the library runs at one priority.
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "creating a stream");
+    *out = new garg_stream{ctx, s};
+    return GARG_OK;
+}
+
+int garg_stream_destroy(garg_stream* stream) {
+    if (!stream) return GARG_OK;
+    (void)hipStreamSynchronize(stream->s);
+    (void)hipStreamDestroy(stream->s);
+    delete stream;
+    return GARG_OK;
+}
+
+int garg_stream_sync(garg_ctx* ctx, garg_stream* stream) {
+    REQUIRE(ctx, "ctx is required.");
+    return sync_stream(stream_of(ctx, stream), "waiting for the stream");
+}
+
+int garg_event_create(garg_ctx* ctx, garg_event** out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e), "creating an event");
+    *out = new garg_event{ctx, e};
+    return GARG_OK;
+}
+
+int garg_event_destroy(garg_event* event) {
+    if (!event) return GARG_OK;
+    (void)hipEventDestroy(event->e);
+    delete event;
+    return GARG_OK;
+}
+
+int garg_event_record(garg_event* event, garg_stream* stream) {
+    REQUIRE(event, "event is required.");
+    HIP_TRY(hipEventRecord(event->e, stream_of(event->ctx, stream)), "recording an event");
+    return GARG_OK;
+}
+
+int garg_event_sync(garg_event* event) {
+    REQUIRE(event, "event is required.");
+    HIP_TRY(hipEventSynchronize(event->e), "waiting for an event");
+    return GARG_OK;
+}
+
+int garg_event_elapsed_ms(garg_event* start, garg_event* end, float* ms) {
+    REQUIRE(start && end && ms, "start, end and ms are required.");
+    HIP_TRY(hipEventElapsedTime(ms, start->e, end->e), "reading elapsed time");
+    return GARG_OK;
+}
+
+// ---- particles -------------------------------------------------------------
+
+int garg_particles_create(garg_ctx* ctx, uint32_t count, uint64_t seed, uint32_t potential, garg_particles** out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    REQUIRE(count > 0, "the particle count must be positive.");
+    REQUIRE(potential <= 1u, "unknown potential %u.", potential);
+    garg_particles* p = new garg_particles();
+    p->ctx = ctx;
+    p->count = count;
+    p->seed = seed;
+    p->potential = potential;
+    uint64_t field = align256((uint64_t)count * 4u);
+    p->bytes = field * 8u;
+    int rc = ctx_alloc(ctx, p->bytes, &p->base, "the particle store");
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    char* b = static_cast<char*>(p->base);
+    p->view = ParticleView{(float*)(b), (float*)(b + field), (float*)(b + 2 * field), (float*)(b + 3 * field),
+                           (float*)(b + 4 * field), (float*)(b + 5 * field), (float*)(b + 6 * field),
+                           (uint32_t*)(b + 7 * field), count};
+    *out = p;
+    return GARG_OK;
+}
+
+int garg_particles_destroy(garg_particles* p) {
+    if (!p) return GARG_OK;
+    (void)hipStreamSynchronize(p->ctx->stream);
+    ctx_free(p->ctx, p->base, p->bytes);
+    delete p;
+    return GARG_OK;
+}
+
+static SimArgs sim_args(garg_particles* p, uint32_t generator, float sigma_z) {
+    SimArgs a;
+    a.p = p->view;
+    a.seed_lo = (uint32_t)p->seed;
+    a.seed_hi = (uint32_t)(p->seed >> 32);
+    a.potential = p->potential;
+    a.generator = generator;
+    a.sigma_z = sigma_z;
+    a.dt = GARG_F(SIM_DT);
+    a.alpha = GARG_F(DRAG_ALPHA);
+    a.temperature_scale = (float)p->ctx->temperature_scale;
+    a.inverse_cdf = p->ctx->inverse_cdf;
+    a.inverse_cdf_count = INVERSE_CDF_COUNT;
+    return a;
+}
+
+int garg_particles_seed(garg_particles* p, garg_stream* stream, uint32_t generator, float sigma_z) {
+    REQUIRE(p, "particles are required.");
+    REQUIRE(generator <= 1u, "unknown generator %u.", generator);
+    REQUIRE(generator == GARG_GENERATOR_DISK || (sigma_z >= 0.0f && std::isfinite(sigma_z)),
+            "sigma_z must be a finite non-negative height.");
+    HIP_TRY(launch_seed(sim_args(p, generator, sigma_z), stream_of(p->ctx, stream)), "launching seedParticles");
+    return GARG_OK;
+}
+
+int garg_particles_step(garg_particles* p, garg_stream* stream, uint32_t dispatches) {
+    REQUIRE(p, "particles are required.");
+    SimArgs a = sim_args(p, GARG_GENERATOR_DISK, 0.0f);
+    hipStream_t s = stream_of(p->ctx, stream);
+    for (uint32_t i = 0; i < dispatches; ++i) HIP_TRY(launch_step(a, s), "launching stepParticles");
+    return GARG_OK;
+}
+
+int garg_particles_reseed(garg_particles* p, uint64_t seed) {
+    REQUIRE(p, "particles are required.");
+    p->seed = seed;
+    return GARG_OK;
+}
+
+int garg_particles_read(garg_particles* p, garg_particle* host, uint32_t count) {
+    REQUIRE(p && host, "particles and host are required.");
+    REQUIRE(count == p->count, "the store holds %u particles, not %u.", p->count, count);
+    garg_particle* tmp = nullptr;
+    int rc = ctx_alloc(p->ctx, (uint64_t)count * sizeof(garg_particle), (void**)&tmp, "the particle readback");
+    if (rc) return rc;
+    hipStream_t s = p->ctx->stream;
+    hipError_t e = launch_pack_aos(p->view, tmp, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, (size_t)count * sizeof(garg_particle), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    ctx_free(p->ctx, tmp, (uint64_t)count * sizeof(garg_particle));
+    HIP_TRY(e, "reading the particles back");
+    return GARG_OK;
+}
+
+int garg_particles_write(garg_particles* p, const garg_particle* host, uint32_t count) {
+    REQUIRE(p && host, "particles and host are required.");
+    REQUIRE(count == p->count, "the store holds %u particles, not %u.", p->count, count);
+    garg_particle* tmp = nullptr;
+    int rc = ctx_alloc(p->ctx, (uint64_t)count * sizeof(garg_particle), (void**)&tmp, "the particle upload");
+    if (rc) return rc;
+    hipStream_t s = p->ctx->stream;
+    hipError_t e = hipMemcpyAsync(tmp, host, (size_t)count * sizeof(garg_particle), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_unpack_aos(p->view, tmp, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    ctx_free(p->ctx, tmp, (uint64_t)count * sizeof(garg_particle));
+    HIP_TRY(e, "writing the particles");
+    return GARG_OK;
+}
+
+int garg_particles_checksum(garg_particles* p, uint64_t* fnv1a) {
+    REQUIRE(p && fnv1a, "particles and fnv1a are required.");
+    std::vector<garg_particle> host(p->count);
+    int rc = garg_particles_read(p, host.data(), p->count);
+    if (rc) return rc;
+    const uint8_t* bytes = reinterpret_cast<const uint8_t*>(host.data());
+    uint64_t hash = 0xcbf29ce484222325ull;
+    for (size_t i = 0; i < host.size() * sizeof(garg_particle); ++i) {
+        hash ^= bytes[i];
+        hash *= 0x100000001b3ull;
+    }
+    *fnv1a = hash;
+    return GARG_OK;
+}
+
+// ---- volume ----------------------------------------------------------------
+
+int garg_volume_create(garg_ctx* ctx, uint32_t size, float half_extent, float half_extent_z, garg_volume** out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    REQUIRE(size >= 2 && size <= 1024, "the volume size must be in [2, 1024], not %u.", size);
+    REQUIRE(half_extent > 0.0f && half_extent_z >= 0.0f, "half extents must be positive.");
+    garg_volume* v = new garg_volume();
+    v->ctx = ctx;
+    v->size = size;
+    v->half_xy = half_extent;
+    v->half_z = half_extent_z > 0.0f ? half_extent_z : half_extent;
+    uint64_t voxels = (uint64_t)size * size * size;
+    uint64_t sums = align256(voxels * 24u), fields = align256(voxels * 16u);
+    v->bytes = sums + fields;
+    int rc = ctx_alloc(ctx, v->bytes, &v->base, "the volume");
+    if (rc) {
+        delete v;
+        return rc;
+    }
+    char* b = static_cast<char*>(v->base);
+    v->emission = (uint32_t*)b;
+    v->momentum = (int32_t*)(b + voxels * 8u);
+    v->mass = (uint32_t*)(b + voxels * 20u);
+    v->voxels = (Voxel*)(b + sums);
+    hipError_t e = launch_fill_zero(v->base, v->bytes, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    HIP_TRY(e, "clearing the volume");
+    *out = v;
+    return GARG_OK;
+}
+
+int garg_volume_destroy(garg_volume* v) {
+    if (!v) return GARG_OK;
+    (void)hipStreamSynchronize(v->ctx->stream);
+    ctx_free(v->ctx, v->base, v->bytes);
+    delete v;
+    return GARG_OK;
+}
+
+int garg_volume_clear(garg_volume* v, garg_stream* stream) {
+    REQUIRE(v, "volume is required.");
+    uint64_t voxels = (uint64_t)v->size * v->size * v->size;
+    HIP_TRY(launch_fill_zero(v->base, voxels * 24u, stream_of(v->ctx, stream)), "clearing the sums");
+    return GARG_OK;
+}
+
+int garg_splat_bin(garg_volume* v, garg_particles* p, garg_stream* stream, uint32_t exact, uint32_t frame) {
+    REQUIRE(v && p, "volume and particles are required.");
+    REQUIRE(v->ctx == p->ctx, "volume and particles belong to different contexts.");
+    SplatArgs a;
+    a.p = p->view;
+    a.emission = v->emission;
+    a.momentum = v->momentum;
+    a.mass = v->mass;
+    a.size = v->size;
+    a.half_xy = v->half_xy;
+    a.half_z = v->half_z;
+    a.peak_temperature = GARG_F(T_PEAK_DEFAULT);
+    a.emission_scale = GARG_F(EMISSION_SCALE);
+    a.exact = exact ? 1u : 0u;
+    a.frame = frame;
+    a.seed_lo = (uint32_t)p->seed;
+    a.seed_hi = (uint32_t)(p->seed >> 32);
+    HIP_TRY(launch_splat_bin(a, stream_of(v->ctx, stream)), "launching splatParticles");
+    return GARG_OK;
+}
+
+int garg_splat_resolve(garg_volume* v, garg_stream* stream, uint32_t clear_sums) {
+    REQUIRE(v, "volume is required.");
+    ResolveArgs a;
+    a.emission = v->emission;
+    a.momentum = v->momentum;
+    a.mass = v->mass;
+    a.voxels = v->voxels;
+    a.blackbody = v->ctx->blackbody;
+    a.size = v->size;
+    a.half_xy = v->half_xy;
+    a.half_z = v->half_z;
+    a.peak_temperature = GARG_F(T_PEAK_DEFAULT);
+    a.clear = clear_sums ? 1u : 0u;
+    HIP_TRY(launch_resolve(a, stream_of(v->ctx, stream)), "launching resolveVolume");
+    return GARG_OK;
+}
+
+int garg_volume_read_sums(garg_volume* v, uint32_t* emission, int32_t* momentum, uint32_t* mass) {
+    REQUIRE(v, "volume is required.");
+    uint64_t voxels = (uint64_t)v->size * v->size * v->size;
+    hipStream_t s = v->ctx->stream;
+    if (emission) HIP_TRY(hipMemcpyAsync(emission, v->emission, voxels * 8u, hipMemcpyDeviceToHost, s), "reading sums");
+    if (momentum) HIP_TRY(hipMemcpyAsync(momentum, v->momentum, voxels * 12u, hipMemcpyDeviceToHost, s), "reading sums");
+    if (mass) HIP_TRY(hipMemcpyAsync(mass, v->mass, voxels * 4u, hipMemcpyDeviceToHost, s), "reading sums");
+    return sync_stream(s, "reading sums");
+}
+
+int garg_volume_read_fields(garg_volume* v, uint16_t* emission, uint16_t* velocity) {
+    REQUIRE(v, "volume is required.");
+    uint64_t voxels = (uint64_t)v->size * v->size * v->size;
+    hipStream_t s = v->ctx->stream;
+    if (emission) HIP_TRY(hipMemcpy2DAsync(emission, 8, v->voxels, 16, 8, voxels, hipMemcpyDeviceToHost, s), "reading fields");
+    if (velocity)
+        HIP_TRY(hipMemcpy2DAsync(velocity, 8, (char*)v->voxels + 8, 16, 8, voxels, hipMemcpyDeviceToHost, s),
+                "reading fields");
+    return sync_stream(s, "reading fields");
+}
+
+int garg_volume_write_fields(garg_volume* v, const uint16_t* emission, const uint16_t* velocity) {
+    REQUIRE(v && emission && velocity, "volume, emission and velocity are required.");
+    uint64_t voxels = (uint64_t)v->size * v->size * v->size;
+    hipStream_t s = v->ctx->stream;
+    HIP_TRY(hipMemcpy2DAsync(v->voxels, 16, emission, 8, 8, voxels, hipMemcpyHostToDevice, s), "writing fields");
+    HIP_TRY(hipMemcpy2DAsync((char*)v->voxels + 8, 16, velocity, 8, 8, voxels, hipMemcpyHostToDevice, s),
+            "writing fields");
+    return sync_stream(s, "writing fields");
+}
+
+// ---- sweep tables ------------------------------------------------------------
+
+int garg_tables_create(garg_ctx* ctx, garg_stream* stream, float sphere_radius, garg_tables** out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    REQUIRE(sphere_radius > 4.0f, "the integration sphere must lie outside r = 4.");
+    garg_tables* t = new garg_tables();
+    t->ctx = ctx;
+    t->sphere_radius = sphere_radius;
+    // Rays whose periapsis lies outside r = R - 1 never enter, so no ray is
+    // placed on the sphere grazing it (DeflectionTables.swift:193-202).
+    t->entry_radius = (double)sphere_radius - 1.0;
+    t->camera_radius = 0.0;
+    int rc = ctx_alloc(ctx, 3u * SWEEP_TABLE_SIZE * 4u, (void**)&t->data, "the sweep tables");
+    if (rc) {
+        delete t;
+        return rc;
+    }
+    t->sphere_max_b = (float)max_impact_parameter(t->entry_radius);
+    t->sky_min_b = t->sphere_max_b;
+    t->sphere_table_max_b = (float)max_impact_parameter(t->sphere_radius);
+    HIP_TRY(launch_sweep_table(t->data, SWEEP_TABLE_SIZE, t->sphere_radius, 0.0, max_impact_parameter(t->sphere_radius),
+                               SWEEP_INTERVALS_SPHERE, 0, stream_of(ctx, stream)),
+            "building the sphere sweep table");
+    *out = t;
+    return GARG_OK;
+}
+
+int garg_tables_destroy(garg_tables* t) {
+    if (!t) return GARG_OK;
+    (void)hipStreamSynchronize(t->ctx->stream);
+    ctx_free(t->ctx, t->data, 3u * SWEEP_TABLE_SIZE * 4u);
+    delete t;
+    return GARG_OK;
+}
+
+int garg_tables_update(garg_tables* t, garg_stream* stream, float camera_radius) {
+    REQUIRE(t, "tables are required.");
+    REQUIRE(camera_radius > 3.5f, "the camera radius must exceed 3.5.");
+    double r = (double)camera_radius;
+    if (std::fabs(r - t->camera_radius) <= 1e-6) return GARG_OK;
+    t->camera_radius = r;
+    double bmax = max_impact_parameter(r);
+    t->camera_max_b = (float)bmax;
+    t->sky_max_b = t->camera_max_b;
+    hipStream_t s = stream_of(t->ctx, stream);
+    HIP_TRY(launch_sweep_table(t->data + SWEEP_TABLE_SIZE, SWEEP_TABLE_SIZE, r, 0.0, bmax, SWEEP_INTERVALS_CAMERA, 0, s),
+            "building the camera sweep table");
+    HIP_TRY(launch_sweep_table(t->data + 2 * SWEEP_TABLE_SIZE, SWEEP_TABLE_SIZE, r,
+                               max_impact_parameter(t->entry_radius), bmax, SWEEP_INTERVALS_CAMERA, 1, s),
+            "building the sky sweep table");
+    return GARG_OK;
+}
+
+int garg_tables_apply(garg_tables* t, garg_march_uniforms* u) {
+    REQUIRE(t && u, "tables and uniforms are required.");
+    REQUIRE(t->camera_radius > 0.0, "the tables have no camera radius yet; call garg_tables_update first.");
+    u->sphere_radius = (float)t->sphere_radius;
+    u->camera_radius = (float)t->camera_radius;
+    u->sphere_max_b = t->sphere_max_b;
+    u->sphere_table_max_b = t->sphere_table_max_b;
+    u->camera_max_b = t->camera_max_b;
+    u->sky_min_b = t->sky_min_b;
+    u->sky_max_b = t->sky_max_b;
+    return GARG_OK;
+}
+
+int garg_tables_read(garg_tables* t, float* sphere, float* camera, float* sky) {
+    REQUIRE(t, "tables are required.");
+    hipStream_t s = t->ctx->stream;
+    HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
+    if (sphere) HIP_TRY(hipMemcpyAsync(sphere, t->data, SWEEP_TABLE_SIZE * 4u, hipMemcpyDeviceToHost, s), "reading tables");
+    if (camera)
+        HIP_TRY(hipMemcpyAsync(camera, t->data + SWEEP_TABLE_SIZE, SWEEP_TABLE_SIZE * 4u, hipMemcpyDeviceToHost, s),
+                "reading tables");
+    if (sky)
+        HIP_TRY(hipMemcpyAsync(sky, t->data + 2 * SWEEP_TABLE_SIZE, SWEEP_TABLE_SIZE * 4u, hipMemcpyDeviceToHost, s),
+                "reading tables");
+    return sync_stream(s, "reading tables");
+}
+
+// ---- targets ---------------------------------------------------------------
+
+int garg_targets_create(garg_ctx* ctx, uint32_t width, uint32_t height, garg_targets** out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    REQUIRE(width > 0 && height > 0 && (uint64_t)width * height <= (1ull << 28), "bad target size %ux%u.", width, height);
+    garg_targets* t = new garg_targets();
+    t->ctx = ctx;
+    t->width = width;
+    t->height = height;
+    uint64_t px = (uint64_t)width * height;
+    uint64_t c = align256(px * 8u), dbg = align256(px * 8u), dep = align256(px * 4u), cnt = 256, rgba = align256(px * 4u);
+    t->bytes = c + dbg + dep + cnt + rgba;
+    int rc = ctx_alloc(ctx, t->bytes, &t->base, "the march targets");
+    if (rc) {
+        delete t;
+        return rc;
+    }
+    char* b = static_cast<char*>(t->base);
+    t->color = (h4*)b;
+    t->debug = (h4*)(b + c);
+    t->depth = (float*)(b + c + dbg);
+    t->counters = (garg_march_counters*)(b + c + dbg + dep);
+    t->rgba8 = (uint8_t*)(b + c + dbg + dep + cnt);
+    hipError_t e = launch_fill_zero(t->base, t->bytes, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    HIP_TRY(e, "clearing the targets");
+    *out = t;
+    return GARG_OK;
+}
+
+int garg_targets_destroy(garg_targets* t) {
+    if (!t) return GARG_OK;
+    (void)hipStreamSynchronize(t->ctx->stream);
+    ctx_free(t->ctx, t->base, t->bytes);
+    delete t;
+    return GARG_OK;
+}
+
+int garg_targets_read(garg_targets* t, uint16_t* color, uint16_t* debug, float* depth) {
+    REQUIRE(t, "targets are required.");
+    HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
+    uint64_t px = (uint64_t)t->width * t->height;
+    if (color) HIP_TRY(hipMemcpy(color, t->color, px * 8u, hipMemcpyDeviceToHost), "reading the colour target");
+    if (debug) HIP_TRY(hipMemcpy(debug, t->debug, px * 8u, hipMemcpyDeviceToHost), "reading the debug target");
+    if (depth) HIP_TRY(hipMemcpy(depth, t->depth, px * 4u, hipMemcpyDeviceToHost), "reading the depth target");
+    return GARG_OK;
+}
+
+int garg_targets_read_counters(garg_targets* t, garg_march_counters* out) {
+    REQUIRE(t && out, "targets and out are required.");
+    HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
+    HIP_TRY(hipMemcpy(out, t->counters, sizeof(garg_march_counters), hipMemcpyDeviceToHost), "reading counters");
+    return GARG_OK;
+}
+
+int garg_targets_reset_counters(garg_targets* t, garg_stream* stream) {
+    REQUIRE(t, "targets are required.");
+    HIP_TRY(launch_fill_zero(t->counters, 256, stream_of(t->ctx, stream)), "clearing counters");
+    return GARG_OK;
+}
+
+// ---- march -------------------------------------------------------------------
+
+int garg_march_image(garg_ctx* ctx, garg_stream* stream, const garg_march_uniforms* uniforms, garg_volume* volume,
+                     garg_tables* tables, garg_targets* targets, uint32_t extent_w, uint32_t extent_h,
+                     uint32_t integrator) {
+    REQUIRE(ctx && targets, "ctx and targets are required.");
+    int rc = check_uniforms(uniforms);
+    if (rc) return rc;
+    REQUIRE(integrator <= 1u, "unknown integrator %u.", integrator);
+    REQUIRE(targets->width == uniforms->resolution[0] && targets->height == uniforms->resolution[1],
+            "the targets are %ux%u but the uniforms ask for %ux%u.", targets->width, targets->height,
+            uniforms->resolution[0], uniforms->resolution[1]);
+    REQUIRE(uniforms->tile_origin[0] < uniforms->resolution[0] && uniforms->tile_origin[1] < uniforms->resolution[1],
+            "the tile origin lies outside the image.");
+    REQUIRE(uniforms->spin > 0.0f || uniforms->sphere_radius <= 0.0f || tables,
+            "the uniforms enable the sphere jump but no tables were given.");
+    MarchImageArgs a;
+    a.uniforms = device_uniforms(uniforms, volume);
+    a.scene = make_scene(ctx, volume, tables);
+    a.color = targets->color;
+    a.debug = targets->debug;
+    a.depth = targets->depth;
+    a.counters = targets->counters;
+    a.extent_w = extent_w ? std::min(extent_w, uniforms->resolution[0] - uniforms->tile_origin[0])
+                          : uniforms->resolution[0] - uniforms->tile_origin[0];
+    a.extent_h = extent_h ? std::min(extent_h, uniforms->resolution[1] - uniforms->tile_origin[1])
+                          : uniforms->resolution[1] - uniforms->tile_origin[1];
+    HIP_TRY(launch_march_image(a, integrator, stream_of(ctx, stream)), "launching marchImage");
+    return GARG_OK;
+}
+
+int garg_probe(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
+               const garg_march_uniforms* uniforms, garg_tables* tables, uint32_t integrator,
+               garg_ray_probe_result* results) {
+    REQUIRE(ctx && probes && results, "ctx, probes and results are required.");
+    REQUIRE(count > 0, "no probes.");
+    REQUIRE(integrator <= 1u, "unknown integrator %u.", integrator);
+    REQUIRE(uniforms && uniforms->spin >= 0.0f && uniforms->spin < 1.0f, "spin must be in [0, 1).");
+    REQUIRE(uniforms->geodesic.step_cap <= (1u << 24), "the step cap is above 2^24.");
+    REQUIRE(uniforms->spin > 0.0f || uniforms->sphere_radius <= 0.0f || tables,
+            "the uniforms enable the sphere jump but no tables were given.");
+    uint64_t need = align256((uint64_t)count * sizeof(garg_ray_probe)) + (uint64_t)count * sizeof(garg_ray_probe_result);
+    if (need > ctx->probe_scratch_bytes) {
+        ctx_free(ctx, ctx->probe_scratch, ctx->probe_scratch_bytes);
+        ctx->probe_scratch = nullptr;
+        ctx->probe_scratch_bytes = 0;
+        int rc = ctx_alloc(ctx, need, &ctx->probe_scratch, "the probe buffers");
+        if (rc) return rc;
+        ctx->probe_scratch_bytes = need;
+    }
+    hipStream_t s = stream_of(ctx, stream);
+    ProbeArgs a;
+    a.uniforms = device_uniforms(uniforms, nullptr);
+    a.scene = make_scene(ctx, nullptr, tables);
+    a.probes = (const garg_ray_probe*)ctx->probe_scratch;
+    a.results = (garg_ray_probe_result*)((char*)ctx->probe_scratch + align256((uint64_t)count * sizeof(garg_ray_probe)));
+    a.count = count;
+    HIP_TRY(hipMemcpyAsync((void*)a.probes, probes, (size_t)count * sizeof(garg_ray_probe), hipMemcpyHostToDevice, s),
+            "uploading probes");
+    HIP_TRY(launch_probe(a, integrator, s), "launching probeRays");
+    HIP_TRY(hipMemcpyAsync(results, a.results, (size_t)count * sizeof(garg_ray_probe_result), hipMemcpyDeviceToHost, s),
+            "reading probe results");
+    return sync_stream(s, "waiting for the probes");
+}
+
+uint32_t garg_request_rays(uint32_t width, uint32_t height) { return ((width + 7u) / 8u) * ((height + 7u) / 8u) * 64u; }
+
+int garg_batch_config_default(garg_ctx* ctx, garg_batch_config* out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    out->block = 256;
+    out->waves_per_simd = 4;
+    out->compaction_period = 32;
+    out->refill_threshold = 48;
+    return GARG_OK;
+}
+
+int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_request* requests, uint32_t count,
+                     const garg_batch_config* config, garg_march_counters* total) {
+    REQUIRE(ctx && requests && config, "ctx, requests and config are required.");
+    REQUIRE(count > 0 && count <= GARG_BATCH_MAX_REQUESTS, "a batch holds 1 to %d requests, not %u.",
+            GARG_BATCH_MAX_REQUESTS, count);
+    REQUIRE(config->block == 64 || config->block == 128 || config->block == 256, "block must be 64, 128 or 256.");
+    REQUIRE(config->compaction_period > 0, "the compaction period must be positive.");
+    REQUIRE(config->refill_threshold <= 64, "the refill threshold is a lane count of at most 64.");
+    for (uint32_t i = 0; i < count; ++i) {
+        const garg_march_request& r = requests[i];
+        int rc = check_uniforms(&r.uniforms);
+        if (rc) return fail(rc, "request %u: %s", i, g_error.c_str());
+        REQUIRE(r.targets, "request %u has no targets.", i);
+        REQUIRE(r.targets->width == r.uniforms.resolution[0] && r.targets->height == r.uniforms.resolution[1],
+                "request %u: the targets do not match its resolution.", i);
+        REQUIRE(r.uniforms.spin > 0.0f || r.uniforms.sphere_radius <= 0.0f || r.tables,
+                "request %u enables the sphere jump but has no tables.", i);
+        REQUIRE(r.ray_begin <= garg_request_rays(r.uniforms.resolution[0], r.uniforms.resolution[1]),
+                "request %u starts past its last ray.", i);
+    }
+    hipStream_t s = stream_of(ctx, stream);
+    HIP_TRY(launch_fill_zero(ctx->batch_totals, sizeof(garg_march_counters), s), "clearing batch counters");
+    uint32_t waves = config->waves_per_simd ? config->waves_per_simd : 4u;
+    uint32_t grid = std::max(1u, ctx->info.compute_units * 4u * waves / (config->block / 64u));
+    for (int kerr = 1; kerr >= 0; --kerr) {
+        // The staging buffer is reused only once the previous copy from it ran.
+        HIP_TRY(hipEventSynchronize(ctx->batch_staging_free), "waiting for the staging buffer");
+        uint32_t n = 0;
+        uint64_t queue = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            const garg_march_request& r = requests[i];
+            if ((r.uniforms.spin > 0.0f) != (kerr == 1)) continue;
+            uint32_t all = garg_request_rays(r.uniforms.resolution[0], r.uniforms.resolution[1]);
+            uint32_t rays = all - r.ray_begin;
+            if (r.ray_count) rays = std::min(rays, r.ray_count);
+            if (rays == 0) continue;
+            BatchRequest& b = ctx->batch_staging[n++];
+            b.uniforms = device_uniforms(&r.uniforms, r.volume);
+            b.scene = make_scene(ctx, r.volume, r.uniforms.spin > 0.0f ? nullptr : r.tables);
+            b.color = r.targets->color;
+            b.debug = r.targets->debug;
+            b.depth = r.targets->depth;
+            b.counters = r.targets->counters;
+            b.tiles_x = (r.uniforms.resolution[0] + 7u) / 8u;
+            b.ray_begin = r.ray_begin;
+            b.queue_begin = queue;
+            queue += rays;
+        }
+        if (n == 0) continue;
+        HIP_TRY(hipMemcpyAsync(ctx->batch_requests, ctx->batch_staging, sizeof(BatchRequest) * n, hipMemcpyHostToDevice, s),
+                "uploading the batch");
+        HIP_TRY(hipEventRecord(ctx->batch_staging_free, s), "recording an event");
+        BatchArgs a;
+        a.requests = ctx->batch_requests;
+        a.request_count = n;
+        a.requests_kerr = (uint32_t)kerr;
+        a.total_rays = queue;
+        a.queue = ctx->queue;
+        a.totals = ctx->batch_totals;
+        a.compaction_period = config->compaction_period;
+        a.refill_threshold = config->refill_threshold;
+        HIP_TRY(launch_march_batch(a, grid, config->block, s), "launching the batched march");
+        if (kerr == 1) {
+            // Both passes share the request table; the second upload waits for the first launch.
+            HIP_TRY(hipStreamSynchronize(s), "waiting for the spinning pass");
+        }
+    }
+    if (total) {
+        HIP_TRY(hipMemcpyAsync(total, ctx->batch_totals, sizeof(garg_march_counters), hipMemcpyDeviceToHost, s),
+                "reading batch counters");
+        return sync_stream(s, "waiting for the batch");
+    }
+    return GARG_OK;
+}
+
+// ---- bake / walk ---------------------------------------------------------------
+
+uint64_t garg_bake_bytes(uint32_t width, uint32_t height, uint32_t capacity) {
+    uint64_t rays = (uint64_t)width * height;
+    return align256(rays * capacity * 8u) + align256(rays * 16u) + 256u;
+}
+
+int garg_bake_create(garg_ctx* ctx, uint32_t width, uint32_t height, uint32_t capacity, garg_bake** out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    REQUIRE(width > 0 && height > 0 && capacity > 0 && capacity <= 4096, "bad bake size.");
+    garg_bake* b = new garg_bake();
+    b->ctx = ctx;
+    b->width = width;
+    b->height = height;
+    b->capacity = capacity;
+    b->bytes = garg_bake_bytes(width, height, capacity);
+    int rc = ctx_alloc(ctx, b->bytes, &b->base, "the bake buffers");
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    uint64_t rays = (uint64_t)width * height;
+    char* p = static_cast<char*>(b->base);
+    b->samples = (h4*)p;
+    b->headers = (uint4*)(p + align256(rays * capacity * 8u));
+    b->overflow = (uint32_t*)(p + align256(rays * capacity * 8u) + align256(rays * 16u));
+    hipError_t e = launch_fill_zero(b->overflow, 256, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    HIP_TRY(e, "clearing the bake counter");
+    *out = b;
+    return GARG_OK;
+}
+
+int garg_bake_destroy(garg_bake* b) {
+    if (!b) return GARG_OK;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    ctx_free(b->ctx, b->base, b->bytes);
+    delete b;
+    return GARG_OK;
+}
+
+int garg_bake_rows(garg_bake* b, garg_stream* stream, const garg_march_uniforms* u, garg_tables* tables, uint32_t row0,
+                   uint32_t rows) {
+    REQUIRE(b, "bake is required.");
+    int rc = check_uniforms(u);
+    if (rc) return rc;
+    REQUIRE(u->spin == 0.0f, "the bake strategy integrates Schwarzschild rays only; spin must be 0.");
+    REQUIRE(u->resolution[0] == b->width && u->resolution[1] == b->height, "the bake buffers are %ux%u.", b->width,
+            b->height);
+    REQUIRE(row0 < b->height && rows > 0, "rows [%u, %u) lie outside the image.", row0, row0 + rows);
+    REQUIRE(u->sphere_radius <= 0.0f || tables, "the uniforms enable the sphere jump but no tables were given.");
+    hipStream_t s = stream_of(b->ctx, stream);
+    if (row0 == 0) HIP_TRY(launch_fill_zero(b->overflow, 256, s), "clearing the bake counter");
+    BakeArgs a;
+    a.uniforms = device_uniforms(u, nullptr);
+    a.scene = make_scene(b->ctx, nullptr, tables);
+    a.samples = b->samples;
+    a.headers = b->headers;
+    a.overflow = b->overflow;
+    a.row0 = row0;
+    a.rows = std::min(rows, b->height - row0);
+    a.capacity = b->capacity;
+    HIP_TRY(launch_bake(a, s), "launching bakeRays");
+    return GARG_OK;
+}
+
+int garg_bake_walk(garg_bake* b, garg_stream* stream, const garg_march_uniforms* u, garg_volume* volume,
+                   garg_targets* targets) {
+    REQUIRE(b && targets, "bake and targets are required.");
+    int rc = check_uniforms(u);
+    if (rc) return rc;
+    REQUIRE(targets->width == b->width && targets->height == b->height, "the targets do not match the bake.");
+    REQUIRE(u->resolution[0] == b->width && u->resolution[1] == b->height, "the uniforms do not match the bake.");
+    WalkArgs a;
+    a.uniforms = device_uniforms(u, volume);
+    a.scene = make_scene(b->ctx, volume, nullptr);
+    a.samples = b->samples;
+    a.headers = b->headers;
+    a.color = targets->color;
+    a.depth = targets->depth;
+    a.capacity = b->capacity;
+    HIP_TRY(launch_walk(a, stream_of(b->ctx, stream)), "launching walkRays");
+    return GARG_OK;
+}
+
+int garg_bake_overflow(garg_bake* b, uint32_t* rays) {
+    REQUIRE(b && rays, "bake and rays are required.");
+    HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
+    HIP_TRY(hipMemcpy(rays, b->overflow, 4, hipMemcpyDeviceToHost), "reading the overflow count");
+    return GARG_OK;
+}
+
+int garg_bake_read(garg_bake* b, uint32_t* headers, uint16_t* samples) {
+    REQUIRE(b, "bake is required.");
+    HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
+    uint64_t rays = (uint64_t)b->width * b->height;
+    if (headers) HIP_TRY(hipMemcpy(headers, b->headers, rays * 16u, hipMemcpyDeviceToHost), "reading bake headers");
+    if (samples)
+        HIP_TRY(hipMemcpy(samples, b->samples, rays * b->capacity * 8u, hipMemcpyDeviceToHost), "reading bake samples");
+    return GARG_OK;
+}
+
+// ---- present -------------------------------------------------------------------
+
+int garg_tonemap(garg_targets* t, garg_stream* stream, float exposure, uint32_t debug_view, float drift_budget,
+                 uint8_t* rgba8) {
+    REQUIRE(t && rgba8, "targets and rgba8 are required.");
+    REQUIRE(drift_budget > 0.0f || !debug_view, "the debug view needs a positive drift budget.");
+    hipStream_t s = stream_of(t->ctx, stream);
+    TonemapArgs a{t->color, t->debug, t->rgba8, t->width * t->height, exposure, debug_view ? 1u : 0u, drift_budget};
+    HIP_TRY(launch_tonemap(a, s), "launching the tone curve");
+    HIP_TRY(hipMemcpyAsync(rgba8, t->rgba8, (size_t)t->width * t->height * 4u, hipMemcpyDeviceToHost, s),
+            "reading the tone mapped image");
+    return sync_stream(s, "waiting for the tone curve");
+}
+
+// ---- FP64 kernels ----------------------------------------------------------------
+
+int garg_deflection_table(garg_ctx* ctx, garg_stream* stream, double r_obs, uint32_t count, double b_lo, double b_hi,
+                          uint32_t intervals, double* b_out, double* alpha_out, double* sweep_out) {
+    REQUIRE(ctx && b_out && alpha_out && sweep_out, "ctx and the three outputs are required.");
+    REQUIRE(count >= 2 && count <= (1u << 20), "count must be in [2, 2^20].");
+    REQUIRE(b_lo > B_CRITICAL && b_hi > b_lo, "the impact parameters must lie above 3 sqrt(3) and increase.");
+    REQUIRE(r_obs > 3.0 && intervals > 0, "r_obs must exceed 3 and intervals must be positive.");
+    double* buf = nullptr;
+    int rc = ctx_alloc(ctx, (uint64_t)count * 24u, (void**)&buf, "the deflection table");
+    if (rc) return rc;
+    hipStream_t s = stream_of(ctx, stream);
+    hipError_t e = launch_far_field(buf, buf + count, buf + 2 * count, count, r_obs, b_lo, b_hi, intervals, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(b_out, buf, count * 8u, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(alpha_out, buf + count, count * 8u, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(sweep_out, buf + 2 * count, count * 8u, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    ctx_free(ctx, buf, (uint64_t)count * 24u);
+    HIP_TRY(e, "building the deflection table");
+    return GARG_OK;
+}
+
+int garg_shadow_boundary(garg_ctx* ctx, garg_stream* stream, double spin, double r_obs, double inclination_deg,
+                         uint32_t azimuths, uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap,
+                         double* edge_out, double* xi_out) {
+    REQUIRE(ctx && edge_out, "ctx and edge_out are required.");
+    REQUIRE(spin >= 0.0 && spin < 1.0, "spin must be in [0, 1).");
+    REQUIRE(r_obs > 5.0, "the observer must sit outside r = 5.");
+    REQUIRE(inclination_deg > 0.0 && inclination_deg < 180.0, "the inclination must be in (0, 180) degrees.");
+    REQUIRE(azimuths > 0 && azimuths <= 65536 && rounds > 0 && rounds <= 16, "bad azimuth or round count.");
+    REQUIRE(rays_per_round == 64, "the multi-section runs one wave64 per azimuth: 64 rays per round.");
+    REQUIRE(step_cap > 0 && step_cap <= (1u << 20), "bad step cap.");
+    double* buf = nullptr;
+    int rc = ctx_alloc(ctx, (uint64_t)azimuths * 16u, (void**)&buf, "the shadow boundary");
+    if (rc) return rc;
+    hipStream_t s = stream_of(ctx, stream);
+    hipError_t e = launch_shadow(buf, buf + azimuths, azimuths, spin, r_obs, inclination_deg * M_PI / 180.0, rounds,
+                                 rays_per_round, step_cap, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(edge_out, buf, azimuths * 8u, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && xi_out) e = hipMemcpyAsync(xi_out, buf + azimuths, azimuths * 8u, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    ctx_free(ctx, buf, (uint64_t)azimuths * 16u);
+    HIP_TRY(e, "bisecting the shadow boundary");
+    return GARG_OK;
+}
+
+}  // extern "C"

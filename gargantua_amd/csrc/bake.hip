@@ -1,0 +1,155 @@
+// The bake strategy for gfx950 (bake.metal): for a camera that holds still,
+// each ray's sample points inside the disk slab are integrated once and
+// stored, and every later frame walks the stored samples against the current
+// volume. Samples live in structure-of-arrays slabs, sample k of ray i at
+// k * rays + i, so the lanes of a wave read and write consecutive 8-byte
+// records at every k.
+#include "device_march.hpp"
+#include "launch.hpp"
+
+namespace garg {
+
+GARG_DEV uint32_t pack_half2(float a, float b) {
+    _Float16 ha = (_Float16)a, hb = (_Float16)b;
+    uint16_t ua = __builtin_bit_cast(uint16_t, ha), ub = __builtin_bit_cast(uint16_t, hb);
+    return (uint32_t)ua | ((uint32_t)ub << 16);
+}
+
+GARG_DEV float2 unpack_half2(uint32_t v) {
+    _Float16 a = __builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+    _Float16 b = __builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
+    return make_float2((float)a, (float)b);
+}
+
+// bakeRays (bake.metal:216-291): the still-policy Schwarzschild march that
+// stores half4(position, path since the last sample) at least bake_spacing
+// apart while inside the slab, up to `capacity` samples.
+__global__ __launch_bounds__(64) void bake_kernel(BakeArgs a) {
+    const garg_march_uniforms& u = a.uniforms;
+    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = a.row0 + blockIdx.y * 8u + (threadIdx.x >> 3);
+    if (px >= u.resolution[0] || py >= u.resolution[1] || py >= a.row0 + a.rows) return;
+    const uint32_t rays = u.resolution[0] * u.resolution[1];
+    const uint32_t index = py * u.resolution[0] + px;
+    garg_march_uniforms still = u;
+    still.jitter[0] = still.jitter[1] = 0.0f;
+    f3 dir = camera_direction(still, px, py);
+    Prepared p = prepare_ray(load3(u.camera_position), dir, u, a.scene);
+    if (p.sky_only) {
+        a.headers[index] = make_uint4(0u, 0u, pack_half2(p.sky.x, p.sky.y), pack_half2(p.sky.z, 1.0f));
+        return;
+    }
+    PlaneRay ray = p.ray;
+    RayState carry{0, 0, 0, 0, 0};
+    f3 prev = ray_position(ray);
+    float since = 0.0f, travelled = 0.0f, first_depth = -1.0f;
+    uint32_t count = 0, outcome = GARG_RAY_EXHAUSTED;
+    bool overflowed = false, left_sphere = false;
+    const garg_geodesic_settings& gs = u.geodesic;
+    const float hx = u.volume_half_extent;
+    for (uint32_t k = 0; k < gs.step_cap; ++k) {
+        float h = ray_step_length(prev.z, ray.s.r, gs);
+        RayState before = ray.s;
+        ray_apply(ray.s, carry, ray_increment(ray.s, h), false);
+        if (u.sphere_radius > 0.0f && ray.s.r >= u.sphere_radius && ray.s.rdot > 0.0f && before.r < u.sphere_radius) {
+            float hh = h * (u.sphere_radius - before.r) / (ray.s.r - before.r);
+            ray.s = before;
+            ray_apply(ray.s, carry, ray_increment(before, hh), false);
+        }
+        f3 pos = ray_position(ray);
+        float path = length(pos - prev);
+        since += path;
+        travelled += path;
+        prev = pos;
+        if (ray.s.r <= gs.capture_radius) {
+            outcome = GARG_RAY_CAPTURED;
+            break;
+        }
+        if (fabsf(pos.z) < GARG_F(DISK_SLAB_HALF_HEIGHT) && fabsf(pos.x) < hx && fabsf(pos.y) < hx &&
+            since >= u.bake_spacing) {
+            if (count < a.capacity) {
+                store_half4(&a.samples[(size_t)count * rays + index], pos.x, pos.y, pos.z, since);
+                count += 1u;
+                if (first_depth < 0.0f) first_depth = travelled;
+            } else {
+                overflowed = true;
+            }
+            since = 0.0f;
+        }
+        if (u.sphere_radius > 0.0f && ray.s.r >= u.sphere_radius && ray.s.rdot > 0.0f) {
+            outcome = GARG_RAY_ESCAPED;
+            left_sphere = true;
+            break;
+        }
+        if (ray.s.r >= gs.escape_radius) {
+            outcome = GARG_RAY_ESCAPED;
+            break;
+        }
+    }
+    bool falls = outcome == GARG_RAY_CAPTURED ||
+                 (outcome == GARG_RAY_EXHAUSTED && ray.s.rdot < 0.0f &&
+                  fabsf(ray.angular_momentum) < GARG_F(B_CRITICAL) * ray.energy);
+    f3 fin = left_sphere ? sphere_exit_direction(ray, u, a.scene) : ray_direction(ray);
+    float depth = first_depth >= 0.0f ? first_depth : travelled;
+    a.headers[index] = make_uint4(count, falls ? 1u : 0u, pack_half2(fin.x, fin.y),
+                                  pack_half2(fin.z, fminf(depth / (2.0f * GARG_F(R_ESCAPE)), 1.0f)));
+    if (overflowed) atomicAdd(a.overflow, 1u);
+}
+
+// walkRays (bake.metal:293-343): composite the stored samples against the
+// current volume with g^3 (no beaming floor, as the reference walk does),
+// then the starfield along the stored direction.
+__global__ __launch_bounds__(64) void walk_kernel(WalkArgs a) {
+    const garg_march_uniforms& u = a.uniforms;
+    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    if (px >= u.resolution[0] || py >= u.resolution[1]) return;
+    const uint32_t rays = u.resolution[0] * u.resolution[1];
+    const uint32_t index = py * u.resolution[0] + px;
+    uint4 header = a.headers[index];
+    f3 color{0, 0, 0};
+    float transmittance = 1.0f;
+    f3 prev = load3(u.camera_position);
+    const bool gas = a.scene.volume.voxels != nullptr;
+    for (uint32_t k = 0; k < a.capacity; ++k) {
+        if (k >= header.x || transmittance <= GARG_F(TRANSMITTANCE_FLOOR)) break;
+        h4 stored = a.samples[(size_t)k * rays + index];
+        f3 pos{(float)stored.x, (float)stored.y, (float)stored.z};
+        float path = (float)stored.w;
+        if (gas) {
+            Sample s = sample_volume(a.scene.volume, pos);
+            if (s.density > 0.0f) {
+                float g3 = 1.0f;
+                if (u.redshift != 0u) {
+                    float g = redshift_factor(schw_f(length(pos)), normalize(pos - prev), s.velocity);
+                    g3 = g * g * g;
+                }
+                float alpha = 1.0f - expf(-u.opacity_scale * s.density * path);
+                color = color + s.emission * (transmittance * g3 * path);
+                transmittance *= 1.0f - alpha;
+            }
+        }
+        prev = pos;
+    }
+    float2 xy = unpack_half2(header.z), zd = unpack_half2(header.w);
+    if (header.y == 0u && u.star_brightness > 0.0f) {
+        f3 d = normalize(f3{xy.x, xy.y, zd.x});
+        color = color + starfield(d, u.star_seed, a.scene.blackbody) * (transmittance * u.star_brightness);
+    }
+    store_half4(&a.color[index], color.x, color.y, color.z, 1.0f);
+    if (a.depth) a.depth[index] = zd.y;
+}
+
+hipError_t launch_bake(const BakeArgs& a, hipStream_t stream) {
+    dim3 grid((a.uniforms.resolution[0] + 7u) / 8u, (a.rows + 7u) / 8u);
+    bake_kernel<<<grid, 64, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk(const WalkArgs& a, hipStream_t stream) {
+    dim3 grid((a.uniforms.resolution[0] + 7u) / 8u, (a.uniforms.resolution[1] + 7u) / 8u);
+    walk_kernel<<<grid, 64, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace garg

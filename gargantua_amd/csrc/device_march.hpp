@@ -1,0 +1,720 @@
+// Per-ray geodesic march for gfx950: Schwarzschild in ingoing
+// Eddington-Finkelstein coordinates with the sweep-table jump, and Kerr in
+// Boyer-Lindquist coordinates in two forms (the reference's formulas, and a
+// fast form of this library). Each march is a small resumable state machine,
+// so the one-launch-per-frame kernel runs it to the end in one call and the
+// persistent kernel runs it a compaction period at a time.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace garg {
+
+// ---------------------------------------------------------------------------
+// Schwarzschild (Geodesic.h:9-188). The Eddington-Finkelstein time v is never
+// read by anything the march outputs, so it is not carried.
+
+struct RayState {
+    float r, phi, vdot, rdot, phidot;
+};
+
+struct PlaneRay {
+    RayState s;
+    f3 e1, e2;
+    float energy, angular_momentum;
+};
+
+GARG_DEV float schw_f(float r) { return 1.0f - GARG_F(R_SCHWARZSCHILD) / r; }
+GARG_DEV float ray_energy(const RayState& s) { return schw_f(s.r) * s.vdot - s.rdot; }
+GARG_DEV float ray_angular_momentum(const RayState& s) { return s.r * s.r * s.phidot; }
+
+// Euler-Lagrange equations of L = (1/2)(-f vdot^2 + 2 vdot rdot + r^2 phidot^2).
+GARG_DEV RayState ray_derivative(const RayState& s) {
+    float r = s.r;
+    float r2 = r * r;
+    float vdd = -s.vdot * s.vdot / r2 + r * s.phidot * s.phidot;
+    float rdd = 2.0f * s.rdot * s.vdot / r2 + schw_f(r) * (r * s.phidot * s.phidot - s.vdot * s.vdot / r2);
+    float pdd = -2.0f * s.rdot * s.phidot / r;
+    return RayState{s.rdot, s.phidot, vdd, rdd, pdd};
+}
+
+GARG_DEV RayState ray_axpy(const RayState& s, const RayState& k, float h) {
+    return RayState{s.r + k.r * h, s.phi + k.phi * h, s.vdot + k.vdot * h, s.rdot + k.rdot * h, s.phidot + k.phidot * h};
+}
+
+// Classical RK4 increment (h / 6)((k1 + 2 k2) + (2 k3 + k4)).
+GARG_DEV RayState ray_increment(const RayState& s, float h) {
+    RayState k1 = ray_derivative(s);
+    RayState k2 = ray_derivative(ray_axpy(s, k1, 0.5f * h));
+    RayState k3 = ray_derivative(ray_axpy(s, k2, 0.5f * h));
+    RayState k4 = ray_derivative(ray_axpy(s, k3, h));
+    float w = h / 6.0f;
+    return RayState{((k1.r + k2.r * 2.0f) + (k3.r * 2.0f + k4.r)) * w,
+                    ((k1.phi + k2.phi * 2.0f) + (k3.phi * 2.0f + k4.phi)) * w,
+                    ((k1.vdot + k2.vdot * 2.0f) + (k3.vdot * 2.0f + k4.vdot)) * w,
+                    ((k1.rdot + k2.rdot * 2.0f) + (k3.rdot * 2.0f + k4.rdot)) * w,
+                    ((k1.phidot + k2.phidot * 2.0f) + (k3.phidot * 2.0f + k4.phidot)) * w};
+}
+
+// Knuth's TwoSum: s + y exactly as t + err; err is carried into the next add.
+GARG_DEV void two_sum(float& s, float& carry, float increment) {
+    float y = increment + carry;
+    float t = s + y;
+    float bp = t - s;
+    carry = (s - (t - bp)) + (y - bp);
+    s = t;
+}
+
+GARG_DEV void ray_apply(RayState& s, RayState& carry, const RayState& inc, bool compensated) {
+    if (compensated) {
+        two_sum(s.r, carry.r, inc.r);
+        two_sum(s.phi, carry.phi, inc.phi);
+        two_sum(s.vdot, carry.vdot, inc.vdot);
+        two_sum(s.rdot, carry.rdot, inc.rdot);
+        two_sum(s.phidot, carry.phidot, inc.phidot);
+    } else {
+        s.r += inc.r;
+        s.phi += inc.phi;
+        s.vdot += inc.vdot;
+        s.rdot += inc.rdot;
+        s.phidot += inc.phidot;
+    }
+}
+
+// Static observer launch with local photon energy one (Geodesic.h:63-87).
+GARG_DEV PlaneRay launch_ray(f3 origin, f3 direction) {
+    float r0 = length(origin);
+    f3 e1 = origin / r0;
+    f3 unit = normalize(direction);
+    float dr = dot(unit, e1);
+    f3 perpendicular = unit - e1 * dr;
+    float dphi = length(perpendicular);
+    f3 e2;
+    if (dphi < 1e-6f) {
+        f3 helper = fabsf(e1.x) < 0.9f ? f3{1.0f, 0.0f, 0.0f} : f3{0.0f, 1.0f, 0.0f};
+        e2 = normalize(helper - e1 * dot(helper, e1));
+        dphi = 0.0f;
+    } else {
+        e2 = perpendicular / dphi;
+    }
+    float root = sqrtf(schw_f(r0));
+    PlaneRay ray;
+    ray.s = RayState{r0, 0.0f, (1.0f + dr) / root, root * dr, dphi / r0};
+    ray.e1 = e1;
+    ray.e2 = e2;
+    ray.energy = ray_energy(ray.s);
+    ray.angular_momentum = ray_angular_momentum(ray.s);
+    return ray;
+}
+
+GARG_DEV f3 ray_position(const PlaneRay& ray) {
+    float c, s;
+    sincosf(ray.s.phi, &s, &c);
+    return (ray.e1 * c + ray.e2 * s) * ray.s.r;
+}
+
+// Direction measured by a static observer at the ray's radius (Geodesic.h:94-100).
+GARG_DEV f3 ray_direction(const PlaneRay& ray) {
+    float c, s;
+    sincosf(ray.s.phi, &s, &c);
+    f3 radial = ray.e1 * c + ray.e2 * s;
+    f3 tangential = ray.e2 * c - ray.e1 * s;
+    f3 local = radial * (ray.s.rdot / sqrtf(fmaxf(schw_f(ray.s.r), 1e-6f))) + tangential * (ray.s.r * ray.s.phidot);
+    float len = length(local);
+    return len > 0.0f ? local / len : radial;
+}
+
+// Table coordinate of b: tables sample b = low + (high - low)(1 - (1 - t)^2)
+// at t = i / (N - 1), remapped to texel centres (Geodesic.h:106-110).
+GARG_DEV float sweep_coordinate(float low, float high, float b) {
+    float x = fminf(fmaxf((b - low) / (high - low), 0.0f), 1.0f);
+    float t = 1.0f - sqrtf(1.0f - x);
+    return (t * ((float)SWEEP_TABLE_SIZE - 1.0f) + 0.5f) / (float)SWEEP_TABLE_SIZE;
+}
+
+GARG_DEV float sweep_lookup(const float* table, float low, float high, float b) {
+    return table_linear(table, (int)SWEEP_TABLE_SIZE, sweep_coordinate(low, high, b));
+}
+
+GARG_DEV f3 asymptotic_direction(const PlaneRay& ray, float phi) {
+    float c, s;
+    sincosf(phi, &s, &c);
+    return ray.e1 * c + ray.e2 * s;
+}
+
+// Step length: coarse on empty legs farther from the slab than that step
+// (Geodesic.h:177-181).
+GARG_DEV float ray_step_length(float z, float r, const garg_geodesic_settings& g) {
+    float coarse = fminf(fmaxf(g.empty_step_factor * r, g.step_min), g.empty_step_max);
+    if (fabsf(z) > GARG_F(DISK_SLAB_HALF_HEIGHT) + coarse) return coarse;
+    return fminf(fmaxf(g.step_factor * r, g.step_min), g.step_max);
+}
+
+// Largest relative deviation of E or L from the launch values (Geodesic.h:184-188).
+GARG_DEV float ray_drift(const PlaneRay& ray) {
+    float e = fabsf(ray_energy(ray.s) - ray.energy) / fabsf(ray.energy);
+    float l = fabsf(ray_angular_momentum(ray.s) - ray.angular_momentum) / fmaxf(fabsf(ray.angular_momentum), 1e-3f);
+    return fmaxf(e, l);
+}
+
+struct Prepared {
+    PlaneRay ray;
+    bool sky_only;
+    f3 sky;
+};
+
+// Skip the empty leg from the camera to the integration sphere with the
+// sweep tables (Geodesic.h:130-163).
+GARG_DEV Prepared prepare_ray(f3 origin, f3 direction, const garg_march_uniforms& u, const Scene& sc) {
+    Prepared p;
+    p.ray = launch_ray(origin, direction);
+    p.sky_only = false;
+    p.sky = f3{0, 0, 0};
+    if (u.sphere_radius <= 0.0f || p.ray.s.r <= u.sphere_radius) return p;
+    float b = fabsf(p.ray.angular_momentum) / p.ray.energy;
+    if (p.ray.s.rdot >= 0.0f) {
+        float sweep = sweep_lookup(sc.camera_table, 0.0f, u.camera_max_b, b);
+        p.sky_only = true;
+        p.sky = asymptotic_direction(p.ray, p.ray.s.phi + sweep);
+        return p;
+    }
+    if (b >= u.sphere_max_b) {
+        float sweep = sweep_lookup(sc.sky_table, u.sky_min_b, u.sky_max_b, b);
+        p.sky_only = true;
+        p.sky = asymptotic_direction(p.ray, p.ray.s.phi + sweep);
+        return p;
+    }
+    float sweep = sweep_lookup(sc.sphere_table, 0.0f, u.sphere_table_max_b, b) -
+                  sweep_lookup(sc.camera_table, 0.0f, u.camera_max_b, b);
+    float r = u.sphere_radius;
+    float f = schw_f(r);
+    float E = p.ray.energy, L = p.ray.angular_momentum;
+    float phidot = L / (r * r);
+    float rdot = -sqrtf(fmaxf(E * E - f * L * L / (r * r), 0.0f));
+    float vdot = (E + rdot) / f;
+    p.ray.s = RayState{r, p.ray.s.phi + sweep, vdot, rdot, phidot};
+    return p;
+}
+
+GARG_DEV f3 sphere_exit_direction(const PlaneRay& ray, const garg_march_uniforms& u, const Scene& sc) {
+    float b = fabsf(ray.angular_momentum) / ray.energy;
+    float sweep = sweep_lookup(sc.sphere_table, 0.0f, u.sphere_table_max_b, b);
+    return asymptotic_direction(ray, ray.s.phi + sweep);
+}
+
+// The Schwarzschild loop of march.metal:51-117 as a resumable machine.
+struct SchwMarch {
+    PlaneRay ray;
+    RayState carry;
+    Gather g;
+    f3 prev;
+    uint32_t steps;
+    uint32_t outcome;
+    bool left_sphere;
+    bool done;
+
+    GARG_DEV void begin(const PlaneRay& launched) {
+        ray = launched;
+        carry = RayState{0, 0, 0, 0, 0};
+        g.reset();
+        prev = ray_position(ray);
+        steps = 0;
+        outcome = GARG_RAY_EXHAUSTED;
+        left_sphere = false;
+        done = false;
+    }
+
+    // Advance by at most `budget` steps; `gather` false skips the volume.
+    GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget) {
+        const garg_geodesic_settings& gs = u.geodesic;
+        const bool has_volume = gather && sc.volume.voxels != nullptr;
+        for (uint32_t n = 0; n < budget && !done; ++n) {
+            if (steps >= gs.step_cap) {
+                done = true;
+                break;
+            }
+            float h = ray_step_length(prev.z, ray.s.r, gs);
+            RayState before = ray.s, carry_before = carry;
+            ray_apply(ray.s, carry, ray_increment(ray.s, h), compensated);
+            steps += 1;
+            if (u.sphere_radius > 0.0f && ray.s.r >= u.sphere_radius && ray.s.rdot > 0.0f && before.r < u.sphere_radius) {
+                // Land on the sphere: the sweep table is exact only from there.
+                float hh = h * (u.sphere_radius - before.r) / (ray.s.r - before.r);
+                ray.s = before;
+                carry = carry_before;
+                ray_apply(ray.s, carry, ray_increment(before, hh), compensated);
+            }
+            f3 p = ray_position(ray);
+            float path = length(p - prev);
+            g.travelled += path;
+            if (ray.s.r <= gs.capture_radius) {
+                outcome = GARG_RAY_CAPTURED;
+                done = true;
+                break;
+            }
+            if (has_volume) {
+                const PlaneRay& rr = ray;
+                g.sample(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, schw_f(ray.s.r),
+                         [&]() { return ray_direction(rr); });
+            }
+            prev = p;
+            if (u.sphere_radius > 0.0f && ray.s.r >= u.sphere_radius && ray.s.rdot > 0.0f) {
+                outcome = GARG_RAY_ESCAPED;
+                left_sphere = true;
+                done = true;
+                break;
+            }
+            if (ray.s.r >= gs.escape_radius) {
+                outcome = GARG_RAY_ESCAPED;
+                done = true;
+                break;
+            }
+        }
+        if (!done && steps >= gs.step_cap) done = true;
+    }
+
+    // Whether the ray falls into the hole (march.metal:44-47).
+    GARG_DEV bool falls_in() const {
+        return outcome == GARG_RAY_CAPTURED ||
+               (outcome == GARG_RAY_EXHAUSTED && ray.s.rdot < 0.0f &&
+                fabsf(ray.angular_momentum) < GARG_F(B_CRITICAL) * ray.energy);
+    }
+
+    GARG_DEV f3 sky_direction(const garg_march_uniforms& u, const Scene& sc) const {
+        return left_sphere ? sphere_exit_direction(ray, u, sc) : ray_direction(ray);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Kerr (Kerr.h:11-186). E and L are parameters of the separable Hamiltonian;
+// the Carter constant and the null residual are the drift gauges.
+
+struct KerrRayState {
+    float r, theta, phi, pr, ptheta;
+    float energy, angular_momentum;
+};
+
+GARG_DEV float kerr_outer_horizon(float a) { return 1.0f + sqrtf(fmaxf(1.0f - a * a, 0.0f)); }
+GARG_DEV float kerr_delta(float a, float r) { return r * r - 2.0f * r + a * a; }
+GARG_DEV float kerr_sigma(float a, float r, float theta) {
+    float c = cosf(theta);
+    return r * r + a * a * c * c;
+}
+
+// Boyer-Lindquist radius of a Cartesian point (Kerr.h:339-344).
+GARG_DEV float kerr_radius_of(float a, f3 p) {
+    float a2 = a * a;
+    float rho2 = dot(p, p);
+    float halved = 0.5f * (rho2 - a2);
+    return sqrtf(halved + sqrtf(halved * halved + a2 * p.z * p.z));
+}
+
+// Launch from a static observer tetrad along a locally measured direction
+// decomposed against the spherical frame (Kerr.h:348-386).
+GARG_DEV KerrRayState kerr_launch(float a, f3 origin, f3 direction) {
+    float r = kerr_radius_of(a, origin);
+    float rho = length(origin);
+    float theta = acosf(fminf(fmaxf(origin.z / rho, -1.0f), 1.0f));
+    float phi = atan2f(origin.y, origin.x);
+    f3 d = normalize(direction);
+    float st, ct, sp, cp;
+    sincosf(theta, &st, &ct);
+    sincosf(phi, &sp, &cp);
+    f3 r_hat{st * cp, st * sp, ct};
+    f3 theta_hat{ct * cp, ct * sp, -st};
+    f3 phi_hat{-sp, cp, 0.0f};
+    float nr = dot(d, r_hat), nth = dot(d, theta_hat), nph = dot(d, phi_hat);
+    float delta = kerr_delta(a, r);
+    float sigma = r * r + a * a * ct * ct;
+    float sin2 = fmaxf(st * st, 1e-8f);
+    float gtt = -(1.0f - 2.0f * r / sigma);
+    float gtphi = -2.0f * a * r * sin2 / sigma;
+    float gphiphi = (r * r + a * a + 2.0f * a * a * r * sin2 / sigma) * sin2;
+    float grr = sigma / delta;
+    float gthth = sigma;
+    float ut = 1.0f / sqrtf(-gtt);
+    float phi_norm = sqrtf(gphiphi - gtphi * gtphi / gtt);
+    float pt = ut + nph * (-gtphi / gtt) / phi_norm;
+    float pphi = nph / phi_norm;
+    KerrRayState s;
+    s.r = r;
+    s.theta = theta;
+    s.phi = phi;
+    s.energy = -(gtt * pt + gtphi * pphi);
+    s.angular_momentum = gtphi * pt + gphiphi * pphi;
+    s.pr = grr * (nr / sqrtf(grr));
+    s.ptheta = gthth * (nth / sqrtf(gthth));
+    return s;
+}
+
+GARG_DEV f3 kerr_position_sc(float a, float r, float st, float ct, float phi) {
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    float radial = sqrtf(r * r + a * a);
+    return f3{radial * st * cp, radial * st * sp, r * ct};
+}
+
+// Direction of travel against the spherical frame from the coordinate
+// velocities (Kerr.h:396-411).
+GARG_DEV f3 kerr_direction_from(float r, float st, float ct, float phi, float rdot, float thdot, float phdot) {
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    f3 r_hat{st * cp, st * sp, ct};
+    f3 theta_hat{ct * cp, ct * sp, -st};
+    f3 phi_hat{-sp, cp, 0.0f};
+    f3 local = r_hat * rdot + theta_hat * (r * thdot) + phi_hat * (r * st * phdot);
+    float len = length(local);
+    return len > 0.0f ? local / len : r_hat;
+}
+
+// --- reference form: the formulas of Kerr.h as written ---------------------
+namespace kref {
+
+GARG_DEV float2 potentials(float a, const KerrRayState& s) {
+    float p = (s.r * s.r + a * a) * s.energy - a * s.angular_momentum;
+    float sin2 = fmaxf(sinf(s.theta) * sinf(s.theta), 1e-8f);
+    float g = s.angular_momentum * s.angular_momentum / sin2 - 2.0f * a * s.energy * s.angular_momentum +
+              a * a * s.energy * s.energy * sin2;
+    return make_float2(p, g);
+}
+
+GARG_DEV float null_residual(float a, const KerrRayState& s) {
+    float delta = kerr_delta(a, s.r);
+    float2 pg = potentials(a, s);
+    return delta * s.pr * s.pr + s.ptheta * s.ptheta - pg.x * pg.x / delta + pg.y;
+}
+
+GARG_DEV float carter(float a, const KerrRayState& s) {
+    float c = cosf(s.theta);
+    float sin2 = fmaxf(sinf(s.theta) * sinf(s.theta), 1e-8f);
+    float term = s.angular_momentum * s.angular_momentum / sin2 - a * a * s.energy * s.energy;
+    return s.ptheta * s.ptheta + c * c * term;
+}
+
+// Hamilton's equations including the off-shell Sigma terms (Kerr.h:62-99).
+GARG_DEV KerrRayState derivative(float a, const KerrRayState& s) {
+    float r = s.r, theta = s.theta;
+    float e = s.energy, l = s.angular_momentum;
+    float delta = kerr_delta(a, r);
+    float sigma = kerr_sigma(a, r, theta);
+    float delta_prime = 2.0f * r - 2.0f;
+    float st = sinf(theta), ct = cosf(theta);
+    float sin2 = fmaxf(st * st, 1e-8f);
+    float2 pg = potentials(a, s);
+    float p = pg.x, g = pg.y;
+    float f = delta * s.pr * s.pr + s.ptheta * s.ptheta - p * p / delta + g;
+    float hamiltonian = f / (2.0f * sigma);
+    float p_prime = 2.0f * r * e;
+    float fr = delta_prime * s.pr * s.pr - 2.0f * p * p_prime / delta + p * p * delta_prime / (delta * delta);
+    float ftheta = -2.0f * l * l * ct / (sin2 * fmaxf(st, 1e-4f)) + 2.0f * a * a * e * e * st * ct;
+    float sigma_r = 2.0f * r;
+    float sigma_theta = -2.0f * a * a * st * ct;
+    KerrRayState out = s;
+    out.r = delta * s.pr / sigma;
+    out.theta = s.ptheta / sigma;
+    out.phi = (a * p / delta + l / sin2 - a * e) / sigma;
+    out.pr = -(fr / (2.0f * sigma) - hamiltonian * sigma_r / sigma);
+    out.ptheta = -(ftheta / (2.0f * sigma) - hamiltonian * sigma_theta / sigma);
+    return out;
+}
+
+GARG_DEV KerrRayState axpy(KerrRayState s, const KerrRayState& k, float h) {
+    s.r += k.r * h;
+    s.theta += k.theta * h;
+    s.phi += k.phi * h;
+    s.pr += k.pr * h;
+    s.ptheta += k.ptheta * h;
+    return s;
+}
+
+GARG_DEV KerrRayState step(float a, const KerrRayState& s, float h) {
+    KerrRayState k1 = derivative(a, s);
+    KerrRayState k2 = derivative(a, axpy(s, k1, 0.5f * h));
+    KerrRayState k3 = derivative(a, axpy(s, k2, 0.5f * h));
+    KerrRayState k4 = derivative(a, axpy(s, k3, h));
+    KerrRayState out = s;
+    float w = h / 6.0f;
+    out.r += ((k1.r + k2.r * 2.0f) + (k3.r * 2.0f + k4.r)) * w;
+    out.theta += ((k1.theta + k2.theta * 2.0f) + (k3.theta * 2.0f + k4.theta)) * w;
+    out.phi += ((k1.phi + k2.phi * 2.0f) + (k3.phi * 2.0f + k4.phi)) * w;
+    out.pr += ((k1.pr + k2.pr * 2.0f) + (k3.pr * 2.0f + k4.pr)) * w;
+    out.ptheta += ((k1.ptheta + k2.ptheta * 2.0f) + (k3.ptheta * 2.0f + k4.ptheta)) * w;
+    return out;
+}
+
+GARG_DEV f3 position(float a, const KerrRayState& s) {
+    float radial = sqrtf(s.r * s.r + a * a);
+    float st = sinf(s.theta);
+    return f3{radial * st * cosf(s.phi), radial * st * sinf(s.phi), s.r * cosf(s.theta)};
+}
+
+GARG_DEV f3 direction(float a, const KerrRayState& s) {
+    float delta = kerr_delta(a, s.r);
+    float sigma = kerr_sigma(a, s.r, s.theta);
+    float sin2 = fmaxf(sinf(s.theta) * sinf(s.theta), 1e-8f);
+    float rdot = delta * s.pr / sigma;
+    float thdot = s.ptheta / sigma;
+    float2 pg = potentials(a, s);
+    float phdot = (a * pg.x / delta + s.angular_momentum / sin2 - a * s.energy) / sigma;
+    return kerr_direction_from(s.r, sinf(s.theta), cosf(s.theta), s.phi, rdot, thdot, phdot);
+}
+
+// Kerr.h:415-424.
+GARG_DEV float step_length(float a, const KerrRayState& s, const garg_geodesic_settings& g) {
+    float near_h = 0.2f * (s.r - kerr_outer_horizon(a));
+    float pole = fminf(fmaxf(sinf(s.theta) / GARG_F(KERR_POLE_SINE), 0.05f), 1.0f);
+    return fminf(fmaxf(fminf(g.step_factor * s.r, near_h) * pole, GARG_F(KERR_STEP_FLOOR)), g.step_max);
+}
+
+}  // namespace kref
+
+// The Kerr loop of march.metal:123-184 in the reference form.
+struct KerrRefMarch {
+    KerrRayState s;
+    float a, carter0, scale, capture;
+    Gather g;
+    f3 prev;
+    float drift;
+    uint32_t steps;
+    uint32_t outcome;
+    bool done;
+
+    GARG_DEV void begin(float spin, const KerrRayState& launched) {
+        a = spin;
+        s = launched;
+        carter0 = kref::carter(a, s);
+        scale = fmaxf(fabsf(carter0), s.energy * s.energy);
+        capture = kerr_outer_horizon(a) + GARG_F(KERR_CAPTURE_MARGIN);
+        g.reset();
+        prev = kref::position(a, s);
+        drift = 0.0f;
+        steps = 0;
+        outcome = GARG_RAY_EXHAUSTED;
+        done = false;
+    }
+
+    GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, uint32_t budget) {
+        const bool has_volume = gather && sc.volume.voxels != nullptr;
+        for (uint32_t n = 0; n < budget && !done; ++n) {
+            if (steps >= u.geodesic.step_cap) {
+                done = true;
+                break;
+            }
+            s = kref::step(a, s, kref::step_length(a, s, u.geodesic));
+            steps += 1;
+            f3 p = kref::position(a, s);
+            float path = length(p - prev);
+            g.travelled += path;
+            float carter_drift = fabsf(kref::carter(a, s) - carter0) / scale;
+            float null_drift = fabsf(kref::null_residual(a, s)) / (scale * fmaxf(s.r * s.r, 1.0f));
+            drift = fmaxf(drift, fmaxf(carter_drift, null_drift));
+            if (s.r <= capture) {
+                outcome = GARG_RAY_CAPTURED;
+                done = true;
+                break;
+            }
+            if (has_volume) {
+                float f = 1.0f - 2.0f * s.r / kerr_sigma(a, s.r, s.theta);
+                const KerrRayState& ss = s;
+                float aa = a;
+                g.sample(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, fmaxf(f, 0.0f),
+                         [&]() { return kref::direction(aa, ss); });
+            }
+            prev = p;
+            if (s.r >= u.geodesic.escape_radius) {
+                outcome = GARG_RAY_ESCAPED;
+                done = true;
+                break;
+            }
+        }
+        if (!done && steps >= u.geodesic.step_cap) done = true;
+    }
+
+    GARG_DEV bool falls_in() const {
+        return outcome == GARG_RAY_CAPTURED || (outcome == GARG_RAY_EXHAUSTED && s.pr < 0.0f);
+    }
+    GARG_DEV f3 final_direction() const { return kref::direction(a, s); }
+    GARG_DEV f3 final_position() const { return kref::position(a, s); }
+};
+
+// --- fast form ------------------------------------------------------------
+// The same Hamiltonian with per-ray constants folded, one sincos(theta) and
+// reciprocals of Delta, Sigma, sin^2 and max(sin, 1e-4) per evaluation, and
+// the drift gauges, the step length, the metric factor and the direction of
+// a state taken from the evaluation that starts the next step.
+
+struct KerrConsts {
+    float a, a2, E, L, aE, aL, L2, a2E2, two_aEL, rplus;
+};
+
+GARG_DEV KerrConsts kerr_consts(float a, float E, float L) {
+    KerrConsts k;
+    k.a = a;
+    k.a2 = a * a;
+    k.E = E;
+    k.L = L;
+    k.aE = a * E;
+    k.aL = a * L;
+    k.L2 = L * L;
+    k.a2E2 = k.a2 * E * E;
+    k.two_aEL = 2.0f * a * E * L;
+    k.rplus = kerr_outer_horizon(a);
+    return k;
+}
+
+struct KerrEval {
+    float dr, dth, dph, dpr, dpth;  // Hamilton's equations at the state
+    float st, ct;                   // sin, cos of theta
+    float null_residual;            // 2 Sigma H
+    float carter;                   // Carter constant
+    float inv_sigma;
+};
+
+GARG_DEV float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+GARG_DEV KerrEval kerr_eval(const KerrConsts& k, float r, float th, float pr, float pth) {
+    KerrEval e;
+    sincosf(th, &e.st, &e.ct);
+    float r2 = r * r;
+    float delta = (r2 - 2.0f * r) + k.a2;
+    float sigma = r2 + k.a2 * (e.ct * e.ct);
+    float inv_sigma = fast_rcp(sigma);
+    float inv_delta = fast_rcp(delta);
+    float sin2 = fmaxf(e.st * e.st, 1e-8f);
+    float inv_sin2 = fast_rcp(sin2);
+    float inv_ms = fast_rcp(fmaxf(e.st, 1e-4f));
+    float P = (r2 + k.a2) * k.E - k.aL;
+    float P_over_delta = P * inv_delta;
+    float G = (k.L2 * inv_sin2 - k.two_aEL) + k.a2E2 * sin2;
+    float pr2 = pr * pr, pth2 = pth * pth;
+    float F = ((delta * pr2 + pth2) - P * P_over_delta) + G;
+    float H = 0.5f * F * inv_sigma;
+    float delta_prime = 2.0f * r - 2.0f;
+    float fr = (delta_prime * pr2 - 4.0f * r * k.E * P_over_delta) + P_over_delta * P_over_delta * delta_prime;
+    float sc = e.st * e.ct;
+    float fth = -2.0f * k.L2 * e.ct * inv_sin2 * inv_ms + 2.0f * k.a2E2 * sc;
+    e.dr = delta * pr * inv_sigma;
+    e.dth = pth * inv_sigma;
+    e.dph = ((k.a * P_over_delta + k.L * inv_sin2) - k.aE) * inv_sigma;
+    e.dpr = (2.0f * r * H - 0.5f * fr) * inv_sigma;
+    e.dpth = (-2.0f * k.a2 * sc * H - 0.5f * fth) * inv_sigma;
+    e.null_residual = F;
+    e.carter = pth2 + (e.ct * e.ct) * (k.L2 * inv_sin2 - k.a2E2);
+    e.inv_sigma = inv_sigma;
+    return e;
+}
+
+struct KerrFastMarch {
+    KerrConsts k;
+    float r, th, ph, pr, pth;
+    float cr, cth, cph, cpr, cpth;  // TwoSum carries
+    KerrEval e;                      // evaluation at the current state
+    float carter0, scale, capture;
+    Gather g;
+    f3 prev;
+    float drift;
+    uint32_t steps;
+    uint32_t outcome;
+    bool done;
+
+    GARG_DEV void begin(float spin, const KerrRayState& s) {
+        k = kerr_consts(spin, s.energy, s.angular_momentum);
+        r = s.r;
+        th = s.theta;
+        ph = s.phi;
+        pr = s.pr;
+        pth = s.ptheta;
+        cr = cth = cph = cpr = cpth = 0.0f;
+        e = kerr_eval(k, r, th, pr, pth);
+        carter0 = e.carter;
+        scale = fmaxf(fabsf(carter0), k.E * k.E);
+        capture = k.rplus + GARG_F(KERR_CAPTURE_MARGIN);
+        g.reset();
+        prev = kerr_position_sc(k.a, r, e.st, e.ct, ph);
+        drift = 0.0f;
+        steps = 0;
+        outcome = GARG_RAY_EXHAUSTED;
+        done = false;
+    }
+
+    GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget) {
+        const bool has_volume = gather && sc.volume.voxels != nullptr;
+        const garg_geodesic_settings& gs = u.geodesic;
+        for (uint32_t n = 0; n < budget && !done; ++n) {
+            if (steps >= gs.step_cap) {
+                done = true;
+                break;
+            }
+            float near_h = 0.2f * (r - k.rplus);
+            float pole = fminf(fmaxf(e.st / GARG_F(KERR_POLE_SINE), 0.05f), 1.0f);
+            float h = fminf(fmaxf(fminf(gs.step_factor * r, near_h) * pole, GARG_F(KERR_STEP_FLOOR)), gs.step_max);
+            float hh = 0.5f * h;
+            KerrEval k2 = kerr_eval(k, r + e.dr * hh, th + e.dth * hh, pr + e.dpr * hh, pth + e.dpth * hh);
+            KerrEval k3 = kerr_eval(k, r + k2.dr * hh, th + k2.dth * hh, pr + k2.dpr * hh, pth + k2.dpth * hh);
+            KerrEval k4 = kerr_eval(k, r + k3.dr * h, th + k3.dth * h, pr + k3.dpr * h, pth + k3.dpth * h);
+            float w = h / 6.0f;
+            float ir = ((e.dr + k2.dr * 2.0f) + (k3.dr * 2.0f + k4.dr)) * w;
+            float ith = ((e.dth + k2.dth * 2.0f) + (k3.dth * 2.0f + k4.dth)) * w;
+            float iph = ((e.dph + k2.dph * 2.0f) + (k3.dph * 2.0f + k4.dph)) * w;
+            float ipr = ((e.dpr + k2.dpr * 2.0f) + (k3.dpr * 2.0f + k4.dpr)) * w;
+            float ipth = ((e.dpth + k2.dpth * 2.0f) + (k3.dpth * 2.0f + k4.dpth)) * w;
+            if (compensated) {
+                two_sum(r, cr, ir);
+                two_sum(th, cth, ith);
+                two_sum(ph, cph, iph);
+                two_sum(pr, cpr, ipr);
+                two_sum(pth, cpth, ipth);
+            } else {
+                r += ir;
+                th += ith;
+                ph += iph;
+                pr += ipr;
+                pth += ipth;
+            }
+            steps += 1;
+            e = kerr_eval(k, r, th, pr, pth);
+            f3 p = kerr_position_sc(k.a, r, e.st, e.ct, ph);
+            float path = length(p - prev);
+            g.travelled += path;
+            float carter_drift = fabsf(e.carter - carter0) / scale;
+            float null_drift = fabsf(e.null_residual) / (scale * fmaxf(r * r, 1.0f));
+            drift = fmaxf(drift, fmaxf(carter_drift, null_drift));
+            if (r <= capture) {
+                outcome = GARG_RAY_CAPTURED;
+                done = true;
+                break;
+            }
+            if (has_volume) {
+                float f = 1.0f - 2.0f * r * e.inv_sigma;
+                const KerrEval& ee = e;
+                float rr = r, pp = ph;
+                g.sample(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, fmaxf(f, 0.0f),
+                         [&]() { return kerr_direction_from(rr, ee.st, ee.ct, pp, ee.dr, ee.dth, ee.dph); });
+            }
+            prev = p;
+            if (r >= gs.escape_radius) {
+                outcome = GARG_RAY_ESCAPED;
+                done = true;
+                break;
+            }
+        }
+        if (!done && steps >= gs.step_cap) done = true;
+    }
+
+    GARG_DEV bool falls_in() const {
+        return outcome == GARG_RAY_CAPTURED || (outcome == GARG_RAY_EXHAUSTED && pr < 0.0f);
+    }
+    GARG_DEV f3 final_direction() const { return kerr_direction_from(r, e.st, e.ct, ph, e.dr, e.dth, e.dph); }
+    GARG_DEV f3 final_position() const { return kerr_position_sc(k.a, r, e.st, e.ct, ph); }
+};
+
+// Colour of a finished ray: gas plus the starfield behind it unless it falls
+// in (march.metal:226-229, 261-263).
+GARG_DEV f3 finish_color(const Gather& g, bool falls_in, f3 sky, const garg_march_uniforms& u, const Scene& sc) {
+    f3 color = g.color;
+    if (!falls_in && u.star_brightness > 0.0f)
+        color = color + starfield(sky, u.star_seed, sc.blackbody) * (g.transmittance * u.star_brightness);
+    return color;
+}
+
+}  // namespace garg

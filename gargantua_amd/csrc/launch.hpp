@@ -1,0 +1,164 @@
+// Kernel argument blocks and the launchers each .hip file provides to the
+// C-ABI layer (api.cpp). Arguments travel by value in the kernarg segment.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gargantua.h"
+#include "device_common.hpp"
+
+namespace garg {
+
+struct MarchImageArgs {
+    garg_march_uniforms uniforms;
+    Scene scene;
+    h4* color;
+    h4* debug;
+    float* depth;
+    garg_march_counters* counters;
+    uint32_t extent_w, extent_h;
+};
+
+struct ProbeArgs {
+    garg_march_uniforms uniforms;
+    Scene scene;
+    const garg_ray_probe* probes;
+    garg_ray_probe_result* results;
+    uint32_t count;
+};
+
+hipError_t launch_march_image(const MarchImageArgs& a, uint32_t integrator, hipStream_t stream);
+hipError_t launch_probe(const ProbeArgs& a, uint32_t integrator, hipStream_t stream);
+
+// One request of the persistent march as the device sees it.
+struct BatchRequest {
+    garg_march_uniforms uniforms;
+    Scene scene;
+    h4* color;
+    h4* debug;
+    float* depth;
+    garg_march_counters* counters;
+    uint32_t tiles_x;     // 8 x 8 tiles per row of the image
+    uint32_t ray_begin;   // first ray of this request, tile order
+    uint64_t queue_begin; // offset of this request in the batch's ray queue
+};
+
+// Requests of one persistent launch; the LDS counter table bounds their number.
+#define GARG_BATCH_MAX_REQUESTS 256
+
+struct BatchArgs {
+    const BatchRequest* requests;   // sorted by queue_begin, first at 0
+    uint32_t request_count;
+    uint32_t requests_kerr;         // 1: every request has spin > 0; 0: every request has spin 0
+    uint64_t total_rays;
+    unsigned long long* queue;      // eight {head, end} pairs, 128 bytes apart
+    garg_march_counters* totals;
+    uint32_t compaction_period;
+    uint32_t refill_threshold;
+};
+
+hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
+
+// Particles, structure of arrays.
+struct ParticleView {
+    float* px;
+    float* py;
+    float* pz;
+    float* vx;
+    float* vy;
+    float* vz;
+    float* temperature;
+    uint32_t* respawns;
+    uint32_t count;
+};
+
+struct SimArgs {
+    ParticleView p;
+    uint32_t seed_lo, seed_hi;
+    uint32_t potential;
+    uint32_t generator;
+    float sigma_z;
+    float dt, alpha, temperature_scale;
+    const float* inverse_cdf;
+    uint32_t inverse_cdf_count;
+};
+
+hipError_t launch_seed(const SimArgs& a, hipStream_t stream);
+hipError_t launch_step(const SimArgs& a, hipStream_t stream);
+hipError_t launch_pack_aos(const ParticleView& p, garg_particle* out, hipStream_t stream);
+hipError_t launch_unpack_aos(const ParticleView& p, const garg_particle* in, hipStream_t stream);
+
+struct SplatArgs {
+    ParticleView p;
+    uint32_t* emission;   // 2 per voxel
+    int32_t* momentum;    // 3 per voxel
+    uint32_t* mass;       // 1 per voxel
+    uint32_t size;
+    float half_xy, half_z;
+    float peak_temperature, emission_scale;
+    uint32_t exact, frame;
+    uint32_t seed_lo, seed_hi;
+};
+
+struct ResolveArgs {
+    uint32_t* emission;
+    int32_t* momentum;
+    uint32_t* mass;
+    Voxel* voxels;
+    const float4* blackbody;
+    uint32_t size;
+    float half_xy, half_z;
+    float peak_temperature;
+    uint32_t clear;
+};
+
+hipError_t launch_splat_bin(const SplatArgs& a, hipStream_t stream);
+hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
+hipError_t launch_fill_zero(void* ptr, uint64_t bytes, hipStream_t stream);
+
+// FP64 sweep tables: table[i] for the sweep from `radius` over b at table
+// coordinate i / (n - 1) between low and high; kind 0 = sweep to infinity,
+// kind 1 = full escape sweep (turn plus outgoing leg).
+hipError_t launch_sweep_table(float* out, uint32_t n, double radius, double low, double high, uint32_t intervals,
+                              int kind, hipStream_t stream);
+hipError_t launch_far_field(double* b, double* alpha, double* sweep, uint32_t n, double r_obs, double b_lo, double b_hi,
+                            uint32_t intervals, hipStream_t stream);
+hipError_t launch_shadow(double* edge, double* xi, uint32_t azimuths, double spin, double r_obs, double inclination,
+                         uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap, hipStream_t stream);
+
+struct TonemapArgs {
+    const h4* color;
+    const h4* debug;
+    uint8_t* out;
+    uint32_t pixels;
+    float exposure;
+    uint32_t debug_view;
+    float drift_budget;
+};
+hipError_t launch_tonemap(const TonemapArgs& a, hipStream_t stream);
+
+struct BakeArgs {
+    garg_march_uniforms uniforms;
+    Scene scene;
+    h4* samples;         // capacity x rays, sample k of ray i at k * rays + i
+    uint4* headers;      // count, falls in, packed direction xy, packed z + depth
+    uint32_t* overflow;
+    uint32_t row0, rows;
+    uint32_t capacity;
+};
+
+struct WalkArgs {
+    garg_march_uniforms uniforms;
+    Scene scene;
+    const h4* samples;
+    const uint4* headers;
+    h4* color;
+    float* depth;
+    uint32_t capacity;
+};
+
+hipError_t launch_bake(const BakeArgs& a, hipStream_t stream);
+hipError_t launch_walk(const WalkArgs& a, hipStream_t stream);
+
+}  // namespace garg

@@ -1,0 +1,172 @@
+// The one-launch-per-frame march (marchImage, march.metal:194-288) and the
+// validation probes (probeRays, march.metal:290-341) for gfx950. One thread
+// per ray; an 8 x 8 pixel tile is exactly one wave64.
+#include "device_march.hpp"
+#include "launch.hpp"
+
+namespace garg {
+
+// Shade one camera ray of a pixel with the selected integrator form. The
+// result is what marchImage writes to its colour, debug and depth targets.
+struct PixelResult {
+    f3 color;
+    float drift;
+    uint32_t steps;
+    uint32_t outcome;
+    float depth;
+};
+
+template <int Form>
+GARG_DEV PixelResult shade_pixel(const garg_march_uniforms& u, const Scene& sc, uint32_t px, uint32_t py) {
+    PixelResult out;
+    f3 dir = camera_direction(u, px, py);
+    f3 origin = load3(u.camera_position);
+    const uint32_t cap = u.geodesic.step_cap;
+    if (u.spin > 0.0f) {
+        KerrRayState launched = kerr_launch(u.spin, origin, dir);
+        if (Form == GARG_INTEGRATOR_REFERENCE) {
+            KerrRefMarch m;
+            m.begin(u.spin, launched);
+            m.run(u, sc, true, cap);
+            out.color = finish_color(m.g, m.falls_in(), m.final_direction(), u, sc);
+            out.drift = m.drift;
+            out.steps = m.steps;
+            out.outcome = m.outcome;
+            out.depth = fminf(m.g.depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f);
+        } else {
+            KerrFastMarch m;
+            m.begin(u.spin, launched);
+            m.run(u, sc, true, u.compensated != 0u, cap);
+            out.color = finish_color(m.g, m.falls_in(), m.final_direction(), u, sc);
+            out.drift = m.drift;
+            out.steps = m.steps;
+            out.outcome = m.outcome;
+            out.depth = fminf(m.g.depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f);
+        }
+        return out;
+    }
+    Prepared p = prepare_ray(origin, dir, u, sc);
+    if (p.sky_only) {
+        out.color = u.star_brightness > 0.0f ? starfield(p.sky, u.star_seed, sc.blackbody) * u.star_brightness
+                                             : f3{0, 0, 0};
+        out.drift = 0.0f;
+        out.steps = 0;
+        out.outcome = GARG_RAY_ESCAPED;
+        out.depth = 1.0f;
+        return out;
+    }
+    SchwMarch m;
+    m.begin(p.ray);
+    m.run(u, sc, true, Form == GARG_INTEGRATOR_FAST && u.compensated != 0u, cap);
+    bool in = m.falls_in();
+    out.color = finish_color(m.g, in, in ? f3{0, 0, 0} : m.sky_direction(u, sc), u, sc);
+    out.drift = ray_drift(m.ray);
+    out.steps = m.steps;
+    out.outcome = m.outcome;
+    out.depth = fminf(m.g.depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f);
+    return out;
+}
+
+template <int Form>
+__global__ __launch_bounds__(64) void march_image_kernel(MarchImageArgs a) {
+    const garg_march_uniforms& u = a.uniforms;
+    uint32_t gx = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t gy = blockIdx.y * 8u + (threadIdx.x >> 3);
+    uint32_t px = u.tile_origin[0] + gx, py = u.tile_origin[1] + gy;
+    bool active = gx < a.extent_w && gy < a.extent_h && px < u.resolution[0] && py < u.resolution[1];
+    PixelResult r{f3{0, 0, 0}, 0.0f, 0u, GARG_RAY_ESCAPED, 1.0f};
+    if (active) {
+        r = shade_pixel<Form>(u, a.scene, px, py);
+        size_t index = (size_t)py * u.resolution[0] + px;
+        store_half4(&a.color[index], r.color.x, r.color.y, r.color.z, 1.0f);
+        store_half4(&a.debug[index], r.drift, (float)r.steps, (float)r.outcome, 1.0f);
+        if (a.depth) a.depth[index] = r.depth;
+    }
+    add_counters(a.counters, active, r.drift > u.drift_budget, r.outcome == GARG_RAY_EXHAUSTED,
+                 r.outcome == GARG_RAY_CAPTURED, r.steps);
+}
+
+template <int Form>
+__global__ __launch_bounds__(64) void probe_kernel(ProbeArgs a) {
+    uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    if (i >= a.count) return;
+    const garg_march_uniforms& u = a.uniforms;
+    const Scene& sc = a.scene;
+    const garg_ray_probe probe = a.probes[i];
+    f3 origin = load3(probe.origin), dir = load3(probe.direction);
+    garg_ray_probe_result out;
+    f3 pos, d;
+    if (u.spin > 0.0f) {
+        KerrRayState launched = kerr_launch(u.spin, origin, dir);
+        if (Form == GARG_INTEGRATOR_REFERENCE) {
+            KerrRefMarch m;
+            m.begin(u.spin, launched);
+            m.run(u, sc, false, u.geodesic.step_cap);
+            pos = m.final_position();
+            d = m.final_direction();
+            out.drift = m.drift;
+            out.outcome = m.outcome;
+            out.steps = m.steps;
+        } else {
+            KerrFastMarch m;
+            m.begin(u.spin, launched);
+            m.run(u, sc, false, u.compensated != 0u, u.geodesic.step_cap);
+            pos = m.final_position();
+            d = m.final_direction();
+            out.drift = m.drift;
+            out.outcome = m.outcome;
+            out.steps = m.steps;
+        }
+        out.energy = launched.energy;
+        out.angular_momentum = launched.angular_momentum;
+    } else {
+        Prepared p = prepare_ray(origin, dir, u, sc);
+        if (p.sky_only) {
+            pos = ray_position(p.ray);
+            d = p.sky;
+            out.drift = 0.0f;
+            out.outcome = GARG_RAY_ESCAPED;
+            out.steps = 0;
+            out.energy = p.ray.energy;
+            out.angular_momentum = p.ray.angular_momentum;
+        } else {
+            SchwMarch m;
+            m.begin(p.ray);
+            m.run(u, sc, false, Form == GARG_INTEGRATOR_FAST && u.compensated != 0u, u.geodesic.step_cap);
+            pos = ray_position(m.ray);
+            d = (m.outcome == GARG_RAY_ESCAPED && !m.falls_in()) ? m.sky_direction(u, sc) : ray_direction(m.ray);
+            out.drift = ray_drift(m.ray);
+            out.outcome = m.outcome;
+            out.steps = m.steps;
+            out.energy = ray_energy(m.ray.s);
+            out.angular_momentum = ray_angular_momentum(m.ray.s);
+        }
+    }
+    out.position[0] = pos.x;
+    out.position[1] = pos.y;
+    out.position[2] = pos.z;
+    out.direction[0] = d.x;
+    out.direction[1] = d.y;
+    out.direction[2] = d.z;
+    a.results[i] = out;
+}
+
+hipError_t launch_march_image(const MarchImageArgs& a, uint32_t integrator, hipStream_t stream) {
+    dim3 grid((a.extent_w + 7u) / 8u, (a.extent_h + 7u) / 8u);
+    if (integrator == GARG_INTEGRATOR_REFERENCE)
+        march_image_kernel<GARG_INTEGRATOR_REFERENCE><<<grid, 64, 0, stream>>>(a);
+    else
+        march_image_kernel<GARG_INTEGRATOR_FAST><<<grid, 64, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe(const ProbeArgs& a, uint32_t integrator, hipStream_t stream) {
+    dim3 grid((a.count + 63u) / 64u);
+    if (integrator == GARG_INTEGRATOR_REFERENCE)
+        probe_kernel<GARG_INTEGRATOR_REFERENCE><<<grid, 64, 0, stream>>>(a);
+    else
+        probe_kernel<GARG_INTEGRATOR_FAST><<<grid, 64, 0, stream>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace garg

@@ -1,0 +1,236 @@
+// The batched persistent march for gfx950. Rays of many render requests,
+// each in 8 x 8 tile order, form one queue split into eight contiguous
+// ranges, one per XCD (workgroups are dispatched round-robin over the eight
+// XCDs, so blockIdx % 8 names the XCD and its L2). A wave keeps 64 rays in
+// registers, marches them a compaction period at a time, and when at most
+// `refill_threshold` lanes are still live it refills the finished lanes with
+// the next rays of its XCD's range in one atomic claim, stealing from the
+// other ranges once its own is empty (Aila and Laine 2009, persistent
+// threads with dynamic fetch). Every ray is marched by the same per-ray
+// state machine as the one-launch-per-frame kernel, so its result does not
+// depend on which lane, wave or batch carries it.
+#include "device_march.hpp"
+#include "launch.hpp"
+
+namespace garg {
+
+constexpr int XCDS = 8;
+constexpr int QUEUE_STRIDE = 16;  // 128 bytes between queue heads
+
+GARG_DEV uint64_t wave_broadcast(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Request holding global ray g (requests sorted by queue_begin).
+GARG_DEV uint32_t find_request(const BatchArgs& a, uint64_t g) {
+    uint32_t lo = 0, hi = a.request_count;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a.requests[mid].queue_begin <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+struct LaneRay {
+    uint32_t request;
+    uint32_t pixel;  // py * width + px
+};
+
+// Tile-order ray of a request to its pixel; false for the padding rays of
+// edge tiles that fall outside the image.
+GARG_DEV bool ray_pixel(const BatchRequest& r, uint64_t g, uint32_t& px, uint32_t& py) {
+    uint32_t local = (uint32_t)(g - r.queue_begin) + r.ray_begin;
+    uint32_t tile = local >> 6, lane = local & 63u;
+    px = (tile % r.tiles_x) * 8u + (lane & 7u);
+    py = (tile / r.tiles_x) * 8u + (lane >> 3);
+    return px < r.uniforms.resolution[0] && py < r.uniforms.resolution[1];
+}
+
+struct CounterTable {
+    uint32_t* counts;  // [request][4]: rays, over budget, exhausted, captured
+    unsigned long long* steps;
+};
+
+GARG_DEV void finish_ray(const BatchRequest& r, CounterTable& t, uint32_t request, uint32_t pixel, f3 color,
+                         float drift, uint32_t steps, uint32_t outcome, float depth) {
+    store_half4(&r.color[pixel], color.x, color.y, color.z, 1.0f);
+    store_half4(&r.debug[pixel], drift, (float)steps, (float)outcome, 1.0f);
+    if (r.depth) r.depth[pixel] = depth;
+    uint32_t* c = t.counts + 4u * request;
+    atomicAdd(&c[0], 1u);
+    if (drift > r.uniforms.drift_budget) atomicAdd(&c[1], 1u);
+    if (outcome == GARG_RAY_EXHAUSTED) atomicAdd(&c[2], 1u);
+    if (outcome == GARG_RAY_CAPTURED) atomicAdd(&c[3], 1u);
+    if (steps) atomicAdd(&t.steps[request], (unsigned long long)steps);
+}
+
+template <bool KERR>
+struct Lane;
+
+template <>
+struct Lane<true> {
+    KerrFastMarch m;
+    GARG_DEV bool start(const BatchRequest& r, uint32_t px, uint32_t py, CounterTable&, uint32_t, uint32_t) {
+        const garg_march_uniforms& u = r.uniforms;
+        f3 dir = camera_direction(u, px, py);
+        m.begin(u.spin, kerr_launch(u.spin, load3(u.camera_position), dir));
+        return true;
+    }
+    GARG_DEV void run(const BatchRequest& r, uint32_t budget) {
+        m.run(r.uniforms, r.scene, true, r.uniforms.compensated != 0u, budget);
+    }
+    GARG_DEV bool done() const { return m.done; }
+    GARG_DEV void finish(const BatchRequest& r, CounterTable& t, uint32_t request, uint32_t pixel) {
+        f3 color = finish_color(m.g, m.falls_in(), m.final_direction(), r.uniforms, r.scene);
+        finish_ray(r, t, request, pixel, color, m.drift, m.steps, m.outcome,
+                   fminf(m.g.depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f));
+    }
+};
+
+template <>
+struct Lane<false> {
+    SchwMarch m;
+    // Rays that never enter the integration sphere finish at the jump.
+    GARG_DEV bool start(const BatchRequest& r, uint32_t px, uint32_t py, CounterTable& t, uint32_t request,
+                        uint32_t pixel) {
+        const garg_march_uniforms& u = r.uniforms;
+        f3 dir = camera_direction(u, px, py);
+        Prepared p = prepare_ray(load3(u.camera_position), dir, u, r.scene);
+        if (p.sky_only) {
+            f3 color = u.star_brightness > 0.0f ? starfield(p.sky, u.star_seed, r.scene.blackbody) * u.star_brightness
+                                                : f3{0, 0, 0};
+            finish_ray(r, t, request, pixel, color, 0.0f, 0u, GARG_RAY_ESCAPED, 1.0f);
+            return false;
+        }
+        m.begin(p.ray);
+        return true;
+    }
+    GARG_DEV void run(const BatchRequest& r, uint32_t budget) {
+        m.run(r.uniforms, r.scene, true, r.uniforms.compensated != 0u, budget);
+    }
+    GARG_DEV bool done() const { return m.done; }
+    GARG_DEV void finish(const BatchRequest& r, CounterTable& t, uint32_t request, uint32_t pixel) {
+        bool in = m.falls_in();
+        f3 color = finish_color(m.g, in, in ? f3{0, 0, 0} : m.sky_direction(r.uniforms, r.scene), r.uniforms, r.scene);
+        finish_ray(r, t, request, pixel, color, ray_drift(m.ray), m.steps, m.outcome,
+                   fminf(m.g.depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f));
+    }
+};
+
+template <bool KERR, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void march_batch_kernel(BatchArgs a) {
+    __shared__ uint32_t s_counts[GARG_BATCH_MAX_REQUESTS * 4];
+    __shared__ unsigned long long s_steps[GARG_BATCH_MAX_REQUESTS];
+    for (uint32_t i = threadIdx.x; i < a.request_count * 4u; i += BLOCK) s_counts[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < a.request_count; i += BLOCK) s_steps[i] = 0ull;
+    __syncthreads();
+    CounterTable table{s_counts, s_steps};
+
+    const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t home = blockIdx.x % XCDS;
+    uint32_t queue = home;
+    bool drained = false;
+
+    Lane<KERR> ray;
+    bool live = false;
+    LaneRay id{0u, 0u};
+
+    for (;;) {
+        uint64_t dead = __ballot(!live);
+        uint32_t live_count = 64u - (uint32_t)__popcll(dead);
+        if (!drained && dead != 0ull && live_count <= a.refill_threshold) {
+            // Claim up to one ray per dead lane, stealing once home is empty.
+            uint64_t want = (uint64_t)__popcll(dead);
+            uint64_t base = 0, got = 0;
+            while (!drained && got == 0) {
+                uint64_t b = 0;
+                if (lane == 0) b = atomicAdd(&a.queue[queue * QUEUE_STRIDE], (unsigned long long)want);
+                b = wave_broadcast(b);
+                uint64_t end = a.queue[queue * QUEUE_STRIDE + 1];
+                if (b < end) {
+                    base = b;
+                    got = min(want, end - b);
+                } else {
+                    queue = (queue + 1u) % XCDS;
+                    if (queue == home) drained = true;
+                }
+            }
+            uint64_t rank = (uint64_t)__popcll(dead & below);
+            if (!live && rank < got) {
+                uint64_t g = base + rank;
+                uint32_t r = find_request(a, g);
+                const BatchRequest& req = a.requests[r];
+                uint32_t px, py;
+                if (ray_pixel(req, g, px, py)) {
+                    id = LaneRay{r, py * req.uniforms.resolution[0] + px};
+                    live = ray.start(req, px, py, table, r, id.pixel);
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        if (live) {
+            const BatchRequest& req = a.requests[id.request];
+            ray.run(req, a.compaction_period);
+            if (ray.done()) {
+                ray.finish(req, table, id.request, id.pixel);
+                live = false;
+            }
+        }
+    }
+
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.request_count; i += BLOCK) {
+        garg_march_counters* c = a.requests[i].counters;
+        uint32_t* cc = reinterpret_cast<uint32_t*>(c);
+        uint32_t* tc = reinterpret_cast<uint32_t*>(a.totals);
+        for (int k = 0; k < 4; ++k) {
+            uint32_t v = s_counts[4u * i + k];
+            if (v) {
+                atomicAdd(&cc[k], v);
+                atomicAdd(&tc[k], v);
+            }
+        }
+        if (s_steps[i]) {
+            atomicAdd((unsigned long long*)&c->steps, s_steps[i]);
+            atomicAdd((unsigned long long*)&a.totals->steps, s_steps[i]);
+        }
+    }
+}
+
+__global__ void queue_init_kernel(unsigned long long* queue, uint64_t total) {
+    uint32_t q = threadIdx.x;
+    if (q >= XCDS) return;
+    // Contiguous, tile-aligned ranges.
+    uint64_t tiles = (total + 63u) / 64u;
+    uint64_t begin = min(total, (tiles * q / XCDS) * 64u);
+    uint64_t end = min(total, (tiles * (q + 1u) / XCDS) * 64u);
+    queue[q * QUEUE_STRIDE] = begin;
+    queue[q * QUEUE_STRIDE + 1] = end;
+}
+
+template <bool KERR>
+static hipError_t launch_block(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream) {
+    switch (block) {
+        case 64: march_batch_kernel<KERR, 64><<<grid, 64, 0, stream>>>(a); break;
+        case 128: march_batch_kernel<KERR, 128><<<grid, 128, 0, stream>>>(a); break;
+        case 256: march_batch_kernel<KERR, 256><<<grid, 256, 0, stream>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream) {
+    queue_init_kernel<<<1, 64, 0, stream>>>(a.queue, a.total_rays);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return a.requests_kerr ? launch_block<true>(a, grid, block, stream) : launch_block<false>(a, grid, block, stream);
+}
+
+}  // namespace garg

@@ -1,0 +1,281 @@
+// FP64 kernels for gfx950: the Schwarzschild sweep tables the march jumps
+// with (Deflection.swift, DeflectionTables.swift), a log-spaced far-field
+// deflection table, and the Kerr shadow boundary by parallel multi-section.
+// Double precision stays on these because their stability needs it: the
+// sweep integrand is singular at the turning point and the shadow edge is a
+// separatrix where single precision rays change fate.
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace garg {
+
+GARG_DEV double max_impact_parameter(double r) { return r / sqrt(1.0 - R_SCHWARZSCHILD / r); }
+
+// Table impact parameter at coordinate t (Deflection.swift:45-48).
+GARG_DEV double table_b(double low, double high, double t) {
+    double back = 1.0 - t;
+    return low + (high - low) * (1.0 - back * back);
+}
+
+// Periapsis: smallest root of 1 - b^2 u^2 (1 - 2u) by Newton from u = 1 / b
+// (Deflection.swift:59-69).
+GARG_DEV double periapsis(double b) {
+    double u = 1.0 / b;
+    for (int i = 0; i < 60; ++i) {
+        double value = 1.0 - b * b * u * u * (1.0 - 2.0 * u);
+        double slope = -b * b * (2.0 * u - 6.0 * u * u);
+        double next = u - value / slope;
+        if (fabs(next - u) < 1e-15) {
+            u = next;
+            break;
+        }
+        u = next;
+    }
+    return 1.0 / u;
+}
+
+// Midpoint-rule terms of the sweep from radius r to infinity after
+// u = u_max sin^2(theta) (Deflection.swift:23-39), summed over this thread's
+// share of the intervals and reduced over the workgroup.
+template <int BLOCK>
+GARG_DEV double block_sweep(double r, double b, uint32_t intervals, double* scratch) {
+    double sum = 0.0;
+    if (b > 0.0) {
+        double bound = fmin(b, max_impact_parameter(r));
+        double u_max = 1.0 / r;
+        double h = (M_PI / 2.0) / (double)intervals;
+        for (uint32_t k = threadIdx.x; k < intervals; k += BLOCK) {
+            double theta = ((double)k + 0.5) * h;
+            double s, c;
+            sincos(theta, &s, &c);
+            double u = u_max * s * s;
+            double inner = 1.0 - bound * bound * u * u * (1.0 - 2.0 * u);
+            if (inner > 0.0) sum += bound * 2.0 * u_max * s * c / sqrt(inner);
+        }
+        sum *= h;
+    }
+    scratch[threadIdx.x] = sum;
+    __syncthreads();
+    for (int stride = BLOCK / 2; stride > 0; stride >>= 1) {
+        if ((int)threadIdx.x < stride) scratch[threadIdx.x] += scratch[threadIdx.x + stride];
+        __syncthreads();
+    }
+    double total = scratch[0];
+    __syncthreads();
+    return total;
+}
+
+constexpr int TABLE_BLOCK = 256;
+
+// One workgroup per table entry (DeflectionTables.swift:215-233).
+__global__ __launch_bounds__(TABLE_BLOCK) void sweep_table_kernel(float* out, uint32_t n, double radius, double low,
+                                                                  double high, uint32_t intervals, int kind) {
+    __shared__ double scratch[TABLE_BLOCK];
+    uint32_t i = blockIdx.x;
+    double b = table_b(low, high, (double)i / (double)(n - 1u));
+    double value;
+    if (kind == 0) {
+        value = block_sweep<TABLE_BLOCK>(radius, b, intervals, scratch);
+    } else {
+        // Escape sweep: the turn back to `radius` plus the outgoing leg
+        // (Deflection.swift:77-86).
+        double peri = periapsis(b);
+        double s_peri = block_sweep<TABLE_BLOCK>(peri, b, intervals, scratch);
+        double s_r = block_sweep<TABLE_BLOCK>(radius, b, intervals, scratch);
+        value = 2.0 * (s_peri - s_r) + s_r;
+    }
+    if (threadIdx.x == 0) out[i] = (float)value;
+}
+
+// Far-field table: alpha(b) = 2 S(r_p, b) - pi and S(r_obs, b) at
+// log-spaced b, one workgroup per entry.
+__global__ __launch_bounds__(TABLE_BLOCK) void far_field_kernel(double* b_out, double* alpha, double* sweep, uint32_t n,
+                                                                double r_obs, double b_lo, double b_hi,
+                                                                uint32_t intervals) {
+    __shared__ double scratch[TABLE_BLOCK];
+    uint32_t i = blockIdx.x;
+    double b = b_lo * pow(b_hi / b_lo, (double)i / (double)(n - 1u));
+    double peri = periapsis(b);
+    double s_peri = block_sweep<TABLE_BLOCK>(peri, b, intervals, scratch);
+    double s_obs = block_sweep<TABLE_BLOCK>(r_obs, b, intervals, scratch);
+    if (threadIdx.x == 0) {
+        b_out[i] = b;
+        alpha[i] = 2.0 * s_peri - M_PI;
+        sweep[i] = s_obs;
+    }
+}
+
+// ---- Kerr in double precision (KerrGeodesic.swift:322-514) ----------------
+
+struct KState {
+    double r, theta, phi, pr, ptheta;
+};
+
+struct KParams {
+    double a, E, L;
+};
+
+GARG_DEV KState kd_derivative(const KParams& k, const KState& s) {
+    double a = k.a, e = k.E, l = k.L;
+    double r = s.r;
+    double delta = r * r - 2.0 * r + a * a;
+    double st, ct;
+    sincos(s.theta, &st, &ct);
+    double sigma = r * r + a * a * ct * ct;
+    double sin2 = fmax(st * st, 1e-12);
+    double p = (r * r + a * a) * e - a * l;
+    double g = l * l / sin2 - 2.0 * a * e * l + a * a * e * e * sin2;
+    double f = delta * s.pr * s.pr + s.ptheta * s.ptheta - p * p / delta + g;
+    double hamiltonian = f / (2.0 * sigma);
+    double delta_prime = 2.0 * r - 2.0;
+    double fr = delta_prime * s.pr * s.pr - 2.0 * p * (2.0 * r * e) / delta + p * p * delta_prime / (delta * delta);
+    double ftheta = -2.0 * l * l * ct / (sin2 * fmax(st, 1e-9)) + 2.0 * a * a * e * e * st * ct;
+    KState out;
+    out.r = delta * s.pr / sigma;
+    out.theta = s.ptheta / sigma;
+    out.phi = (a * p / delta + l / sin2 - a * e) / sigma;
+    out.pr = -(fr / (2.0 * sigma) - hamiltonian * 2.0 * r / sigma);
+    out.ptheta = -(ftheta / (2.0 * sigma) - hamiltonian * (-2.0 * a * a * st * ct) / sigma);
+    return out;
+}
+
+GARG_DEV KState kd_axpy(const KState& s, const KState& k, double h) {
+    return KState{s.r + k.r * h, s.theta + k.theta * h, s.phi + k.phi * h, s.pr + k.pr * h, s.ptheta + k.ptheta * h};
+}
+
+GARG_DEV void kd_step(const KParams& k, KState& s, double h) {
+    KState k1 = kd_derivative(k, s);
+    KState k2 = kd_derivative(k, kd_axpy(s, k1, 0.5 * h));
+    KState k3 = kd_derivative(k, kd_axpy(s, k2, 0.5 * h));
+    KState k4 = kd_derivative(k, kd_axpy(s, k3, h));
+    double w = h / 6.0;
+    s.r += (k1.r + k2.r * 2.0 + k3.r * 2.0 + k4.r) * w;
+    s.theta += (k1.theta + k2.theta * 2.0 + k3.theta * 2.0 + k4.theta) * w;
+    s.phi += (k1.phi + k2.phi * 2.0 + k3.phi * 2.0 + k4.phi) * w;
+    s.pr += (k1.pr + k2.pr * 2.0 + k3.pr * 2.0 + k4.pr) * w;
+    s.ptheta += (k1.ptheta + k2.ptheta * 2.0 + k3.ptheta * 2.0 + k4.ptheta) * w;
+}
+
+// Static observer launch in double precision (KerrGeodesic.swift:398-438).
+GARG_DEV void kd_launch(double a, const double o[3], const double dir[3], KState& s, KParams& k) {
+    double a2 = a * a;
+    double rho2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    double half = 0.5 * (rho2 - a2);
+    double r = sqrt(half + sqrt(half * half + a2 * o[2] * o[2]));
+    double rho = sqrt(rho2);
+    double theta = acos(fmin(fmax(o[2] / rho, -1.0), 1.0));
+    double phi = atan2(o[1], o[0]);
+    double dl = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    double d[3] = {dir[0] / dl, dir[1] / dl, dir[2] / dl};
+    double st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
+    double nr = d[0] * st * cp + d[1] * st * sp + d[2] * ct;
+    double nth = d[0] * ct * cp + d[1] * ct * sp - d[2] * st;
+    double nph = -d[0] * sp + d[1] * cp;
+    double delta = r * r - 2.0 * r + a2;
+    double sigma = r * r + a2 * ct * ct;
+    double sin2 = fmax(st * st, 1e-12);
+    double gtt = -(1.0 - 2.0 * r / sigma);
+    double gtphi = -2.0 * a * r * sin2 / sigma;
+    double gphiphi = (r * r + a2 + 2.0 * a2 * r * sin2 / sigma) * sin2;
+    double grr = sigma / delta, gthth = sigma;
+    double ut = 1.0 / sqrt(-gtt);
+    double phi_norm = sqrt(gphiphi - gtphi * gtphi / gtt);
+    double pt = ut + nph * (-gtphi / gtt) / phi_norm;
+    double pphi = nph / phi_norm;
+    k.a = a;
+    k.E = -(gtt * pt + gtphi * pphi);
+    k.L = gtphi * pt + gphiphi * pphi;
+    s = KState{r, theta, phi, grr * (nr / sqrt(grr)), gthth * (nth / sqrt(gthth))};
+}
+
+// 1 when the ray is captured under the oracle's step policy
+// (KerrGeodesic.swift:484-514), 0 when it escapes or runs out of steps.
+GARG_DEV int kd_captured(const KParams& k, KState s, uint32_t step_cap, double escape) {
+    double rplus = 1.0 + sqrt(1.0 - k.a * k.a);
+    double capture = rplus + KERR_CAPTURE_MARGIN;
+    for (uint32_t n = 0; n < step_cap; ++n) {
+        double near_h = 0.2 * (s.r - rplus);
+        double pole = fmin(fmax(sin(s.theta) / KERR_POLE_SINE, 0.05), 1.0);
+        double h = fmin(fmax(fmin(STEP_RADIUS_FACTOR * s.r, near_h) * pole, 0.4 * 0.005), STEP_MAX);
+        kd_step(k, s, h);
+        if (s.r <= capture) return 1;
+        if (s.r >= escape) return 0;
+    }
+    return 0;
+}
+
+// One workgroup per screen azimuth, one lane per ray of a round. Each round
+// spreads the lanes over the current bracket of aim angles, and the bracket
+// shrinks to the interval where captured turns into escaped
+// (Validate.swift:125-150), so each round gains log2(lanes) bits.
+__global__ __launch_bounds__(64) void shadow_kernel(double* edge, double* xi, uint32_t azimuths, double spin,
+                                                    double r_obs, double inclination, uint32_t rounds,
+                                                    uint32_t step_cap) {
+    uint32_t az = blockIdx.x;
+    if (az >= azimuths) return;
+    const uint32_t lanes = 64;
+    double psi = 2.0 * M_PI * (double)az / (double)azimuths;
+    double si = sin(inclination), ci = cos(inclination);
+    double origin[3] = {r_obs * si, 0.0, r_obs * ci};
+    double forward[3] = {-si, 0.0, -ci};
+    // right = forward x z normalized, up = right x forward
+    double right[3] = {0.0, 1.0, 0.0};
+    double up[3] = {right[1] * forward[2] - right[2] * forward[1], right[2] * forward[0] - right[0] * forward[2],
+                    right[0] * forward[1] - right[1] * forward[0]};
+    double sp = sin(psi), cp = cos(psi);
+    double screen[3] = {cp * right[0] + sp * up[0], cp * right[1] + sp * up[1], cp * right[2] + sp * up[2]};
+    double escape = fmax((double)R_ESCAPE, r_obs + 10.0);
+    double low = 0.0, high = 12.0 / r_obs;
+    for (uint32_t round = 0; round < rounds; ++round) {
+        double aim = low + (high - low) * ((double)threadIdx.x + 0.5) / (double)lanes;
+        double ca = cos(aim), sa = sin(aim);
+        double d[3] = {ca * forward[0] + sa * screen[0], ca * forward[1] + sa * screen[1],
+                       ca * forward[2] + sa * screen[2]};
+        KState s;
+        KParams k;
+        kd_launch(spin, origin, d, s, k);
+        int captured = kd_captured(k, s, step_cap, escape);
+        uint64_t escaped = __ballot(!captured);
+        // First escaped lane; lanes below it were captured.
+        int first = escaped ? __ffsll((unsigned long long)escaped) - 1 : (int)lanes;
+        double new_low = first > 0 ? low + (high - low) * ((double)(first - 1) + 0.5) / (double)lanes : low;
+        double new_high =
+            first < (int)lanes ? low + (high - low) * ((double)first + 0.5) / (double)lanes : high;
+        low = new_low;
+        high = new_high;
+    }
+    if (threadIdx.x == 0) {
+        double aim = 0.5 * (low + high);
+        edge[az] = aim;
+        if (xi) {
+            double ca = cos(aim), sa = sin(aim);
+            double d[3] = {ca * forward[0] + sa * screen[0], ca * forward[1] + sa * screen[1],
+                           ca * forward[2] + sa * screen[2]};
+            KState s;
+            KParams k;
+            kd_launch(spin, origin, d, s, k);
+            xi[az] = k.L / k.E;
+        }
+    }
+}
+
+hipError_t launch_sweep_table(float* out, uint32_t n, double radius, double low, double high, uint32_t intervals,
+                              int kind, hipStream_t stream) {
+    sweep_table_kernel<<<dim3(n), TABLE_BLOCK, 0, stream>>>(out, n, radius, low, high, intervals, kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_far_field(double* b, double* alpha, double* sweep, uint32_t n, double r_obs, double b_lo, double b_hi,
+                            uint32_t intervals, hipStream_t stream) {
+    far_field_kernel<<<dim3(n), TABLE_BLOCK, 0, stream>>>(b, alpha, sweep, n, r_obs, b_lo, b_hi, intervals);
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow(double* edge, double* xi, uint32_t azimuths, double spin, double r_obs, double inclination,
+                         uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap, hipStream_t stream) {
+    (void)rays_per_round;  // one wave64 per azimuth: 64 rays per round
+    shadow_kernel<<<dim3(azimuths), 64, 0, stream>>>(edge, xi, azimuths, spin, r_obs, inclination, rounds, step_cap);
+    return hipGetLastError();
+}
+
+}  // namespace garg

@@ -1,0 +1,264 @@
+"""GPU parity of the march: the batched persistent march against the
+one-launch-per-frame march (bitwise: both run the same per-ray state
+machine), every request alone against the same request in a mix (bitwise),
+both integrator forms against the FP64 oracle per pixel, the beaming and
+jump goldens, bake/walk, and the tone curve."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gargantua_amd import api
+from tests import scenes
+from tests.conftest import golden_passes, load_golden
+from tests.test_oracle_geodesic import synthetic_disk
+
+pytestmark = pytest.mark.gpu
+
+_DISK = {}
+
+
+def disk_volume(ctx, size=64):
+    if size not in _DISK:
+        _DISK[size] = synthetic_disk(size, moving=True)
+    e, v = _DISK[size]
+    vol = api.Volume(ctx, size)
+    vol.write_fields(e.astype(np.float32), np.concatenate([v, np.ones(v.shape[:3] + (1,))], -1).astype(np.float32))
+    return vol
+
+
+def kerr_request(ctx, w, h, spin, inclination, distance, cap, volume, fov=60.0, compensated=True):
+    cam = scenes.look_at(distance, inclination, fov)
+    u = api.uniforms(cam, w, h, api.settings("still", step_cap=cap, escape_radius=max(60.0, distance + 10.0)),
+                     spin=spin, compensated=compensated, star_seed=1)
+    return {"uniforms": u, "targets": api.Targets(ctx, w, h), "volume": volume}
+
+
+def schw_request(ctx, w, h, volume, tables, cap=4096):
+    cam = api.orbit_camera()
+    u = api.uniforms(cam, w, h, api.settings("still", step_cap=cap), compensated=True, star_seed=1)
+    tables.update(cam["distance"])
+    tables.apply(u)
+    return {"uniforms": u, "targets": api.Targets(ctx, w, h), "volume": volume, "tables": tables}
+
+
+def render_plain(ctx, req, integrator=api.INTEGRATOR_FAST):
+    t = api.Targets(ctx, req["targets"].width, req["targets"].height)
+    ctx.march_image(req["uniforms"], t, req.get("volume"), req.get("tables"), integrator)
+    out = t.read(), t.counters()
+    t.close()
+    return out
+
+
+def same_bits(a, b):
+    return all(np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x, y.view(np.uint32)
+                              if y.dtype == np.float32 else y) for x, y in zip(a, b))
+
+
+def test_batched_matches_plain_bitwise_kerr(ctx):
+    vol = disk_volume(ctx)
+    req = kerr_request(ctx, 256, 144, 0.9, 60.0, 50.0, 4096, vol)
+    (plain, pc) = render_plain(ctx, req)
+    total = ctx.march_batch([req], ctx.batch_config(compaction_period=8, refill_threshold=40))
+    batched = req["targets"].read()
+    assert same_bits(plain, batched)
+    bc = req["targets"].counters()
+    assert (bc.rays, bc.steps, bc.captured, bc.exhausted, bc.over_drift_budget) == \
+        (pc.rays, pc.steps, pc.captured, pc.exhausted, pc.over_drift_budget)
+    assert total.steps == pc.steps and pc.rays == 256 * 144
+    print(f"rays {pc.rays} steps {pc.steps} captured {pc.captured}")
+
+
+def test_batched_matches_plain_bitwise_schwarzschild_with_jump(ctx):
+    vol = disk_volume(ctx)
+    tables = api.Tables(ctx)
+    req = schw_request(ctx, 192, 108, vol, tables)
+    (plain, pc) = render_plain(ctx, req)
+    ctx.march_batch([req], ctx.batch_config(compaction_period=16, refill_threshold=32))
+    assert same_bits(plain, req["targets"].read())
+    assert req["targets"].counters().steps == pc.steps
+
+
+@pytest.mark.parametrize("block,period,threshold", [(64, 1, 64), (128, 32, 16), (256, 64, 0)])
+def test_each_request_alone_equals_the_mix(ctx, block, period, threshold):
+    """Continuous batching must not change any result: every request of a
+    mixed batch (spins, sizes, budgets, slices) equals its own plain render."""
+    vol = disk_volume(ctx)
+    tables = api.Tables(ctx)
+    reqs = [kerr_request(ctx, 96, 64, 0.5, 80.0, 40.0, 256, vol),
+            kerr_request(ctx, 64, 40, 0.95, 20.0, 90.0, 4096, vol),
+            kerr_request(ctx, 130, 70, 0.3, 45.0, 30.0, 4096, None, compensated=False),
+            schw_request(ctx, 100, 52, vol, tables)]
+    alone = [render_plain(ctx, r)[0] for r in reqs]
+    # Split the third request across two batch entries to exercise ray slices.
+    rays = api.request_rays(130, 70)
+    mix = [reqs[0], reqs[1], dict(reqs[2], ray_begin=0, ray_count=rays // 3),
+           dict(reqs[2], ray_begin=rays // 3, ray_count=0), reqs[3]]
+    ctx.march_batch(mix, ctx.batch_config(block=block, compaction_period=period, refill_threshold=threshold))
+    for r, a in zip(reqs, alone):
+        assert same_bits(a, r["targets"].read())
+    assert reqs[2]["targets"].counters().rays == 130 * 70
+
+
+def _image_vs_oracle(ctx, req, field, integrator, tables=None, samples=400, seed=3):
+    u = req["uniforms"]
+    t = api.Targets(ctx, u.resolution[0], u.resolution[1])
+    ctx.march_image(u, t, req.get("volume"), req.get("tables"), integrator)
+    color, debug, _ = t.read()
+    t.close()
+    rng = np.random.default_rng(seed)
+    xy = np.stack([rng.integers(0, u.resolution[0], samples), rng.integers(0, u.resolution[1], samples)], 1)
+    p = scenes.oracle_params(u, tables)
+    pix, _ = scenes.oracle_pixels(p, xy, field)
+    outcome_ok = steps_close = color_ok = 0
+    for (x, y), o in zip(xy, pix):
+        gpu_outcome = int(round(debug[y, x, 2]))
+        if gpu_outcome == o.outcome:
+            outcome_ok += 1
+            steps_close += abs(debug[y, x, 1] - o.steps) <= max(2, 0.01 * o.steps)
+            want = np.array(o.color[:])
+            got = color[y, x, :3].astype(np.float64)
+            color_ok += np.all(np.abs(got - want) <= 2e-3 + 2e-2 * np.abs(want))
+    return outcome_ok / samples, steps_close / samples, color_ok / samples
+
+
+@pytest.mark.parametrize("integrator", [api.INTEGRATOR_REFERENCE, api.INTEGRATOR_FAST])
+def test_kerr_image_matches_oracle_per_pixel(ctx, integrator):
+    vol = disk_volume(ctx)
+    req = kerr_request(ctx, 160, 90, 0.9, 60.0, 50.0, 4096, vol, compensated=integrator == api.INTEGRATOR_FAST)
+    field = scenes.field_from_volume(vol)
+    outcome, steps, color = _image_vs_oracle(ctx, req, field, integrator)
+    print(f"kerr: outcome {outcome:.3f} steps {steps:.3f} colour {color:.3f}")
+    assert outcome >= 0.99 and steps >= 0.97 and color >= 0.95
+
+
+@pytest.mark.parametrize("integrator", [api.INTEGRATOR_REFERENCE, api.INTEGRATOR_FAST])
+def test_schwarzschild_image_with_jump_matches_oracle_per_pixel(ctx, integrator):
+    vol = disk_volume(ctx)
+    tables = api.Tables(ctx)
+    req = schw_request(ctx, 160, 90, vol, tables)
+    field = scenes.field_from_volume(vol)
+    otables = scenes.oracle_tables(api.orbit_camera()["distance"])
+    outcome, steps, color = _image_vs_oracle(ctx, req, field, integrator, tables=otables)
+    print(f"schwarzschild: outcome {outcome:.3f} steps {steps:.3f} colour {color:.3f}")
+    assert outcome >= 0.99 and steps >= 0.97 and color >= 0.95
+
+
+def test_reference_and_fast_forms_agree(ctx):
+    vol = disk_volume(ctx)
+    req = kerr_request(ctx, 192, 108, 0.9, 60.0, 50.0, 4096, vol, compensated=False)
+    (a, ca) = render_plain(ctx, req, api.INTEGRATOR_REFERENCE)
+    (b, cb) = render_plain(ctx, req, api.INTEGRATOR_FAST)
+    same_outcome = np.mean(a[1][..., 2] == b[1][..., 2])
+    lum = lambda c: 0.2126 * c[..., 0] + 0.7152 * c[..., 1] + 0.0722 * c[..., 2]
+    diff = np.abs(lum(a[0]) - lum(b[0])).sum() / lum(a[0]).sum()
+    print(f"outcome agreement {same_outcome:.4f}, luminance difference {diff:.2e}, steps {ca.steps} vs {cb.steps}")
+    assert same_outcome > 0.995 and diff < 0.01 and abs(ca.steps - cb.steps) / ca.steps < 0.01
+
+
+@pytest.fixture(scope="module")
+def settled_volume(ctx):
+    """Validate.run: settle 1500 substeps from seed 1, exact splat, resolve."""
+    p = api.Particles(ctx, 1_000_000, 1)
+    p.seed()
+    p.step(750)
+    v = api.Volume(ctx, 128)
+    v.clear()
+    v.splat(p, exact=True)
+    v.resolve(clear=True)
+    yield v
+    v.close()
+    p.close()
+
+
+def test_beaming_golden(ctx, settled_volume):
+    g = load_golden("beaming")
+    w, h = int(g["parameters"]["width"]), int(g["parameters"]["height"])
+    cam = api.orbit_camera()
+    u = api.uniforms(cam, w, h, api.settings("still"), stars=False, compensated=True)
+    t = api.Targets(ctx, w, h)
+    ctx.march_image(u, t, settled_volume, None, api.INTEGRATOR_FAST)
+    color, _, _ = t.read()
+    t.close()
+    lum = np.ascontiguousarray((0.2126 * color[..., 0] + 0.7152 * color[..., 1] + 0.0722 * color[..., 2])
+                               .astype(np.float64))
+    gpu_ratio = O.lib.orc_side_ratio(O.ptr(lum), w, h)
+    field = scenes.field_from_volume(settled_volume)
+    oc = O.Camera()
+    oc.position[:], oc.right[:], oc.up[:], oc.forward[:] = (list(map(float, cam[k])) for k in
+                                                            ("position", "right", "up", "forward"))
+    th = math.tan(float(cam["fov_y"]) * 0.5)
+    oc.tan_half_fov[:] = [th * w / h, th]
+    img = np.zeros((h, w, 3))
+    O.lib.orc_render(w, h, C.byref(oc), C.byref(field), C.byref(O.settings("still")), 0.0005, 1, 8, O.ptr(img))
+    olum = np.ascontiguousarray(0.2126 * img[..., 0] + 0.7152 * img[..., 1] + 0.0722 * img[..., 2])
+    oracle_ratio = O.lib.orc_side_ratio(O.ptr(olum), w, h)
+    print(f"gpu {gpu_ratio:.3f} vs oracle {oracle_ratio:.3f}")
+    assert golden_passes(g, gpu_ratio / oracle_ratio)
+
+
+def test_jump_golden(ctx, settled_volume):
+    g = load_golden("jump")
+    w, h = int(g["parameters"]["width"]), int(g["parameters"]["height"])
+    cam = api.orbit_camera()
+    tables = api.Tables(ctx)
+    tables.update(cam["distance"])
+    lums = []
+    for jump in (False, True):
+        u = api.uniforms(cam, w, h, api.settings("still"), stars=False, compensated=True)
+        if jump:
+            tables.apply(u)
+        t = api.Targets(ctx, w, h)
+        ctx.march_image(u, t, settled_volume, tables if jump else None, api.INTEGRATOR_FAST)
+        c = t.read()[0].astype(np.float64)
+        t.close()
+        lums.append(0.2126 * c[..., 0] + 0.7152 * c[..., 1] + 0.0722 * c[..., 2])
+    measured = np.abs(lums[0] - lums[1]).sum() / lums[0].sum()
+    print(f"jump: mean relative difference {measured:.2e}")
+    assert golden_passes(g, measured)
+
+
+def test_bake_walk_matches_march(ctx, settled_volume):
+    w, h = 192, 108
+    cam = api.orbit_camera()
+    tables = api.Tables(ctx)
+    tables.update(cam["distance"])
+    u = api.uniforms(cam, w, h, api.settings("still"), stars=False, redshift=False,
+                     bake_spacing=60.0 / 128 * 0.5)
+    tables.apply(u)
+    bake = api.Bake(ctx, w, h, 96)
+    for row0 in range(0, h, h // 4):
+        bake.rows(u, tables, row0, h // 4)
+    walked = api.Targets(ctx, w, h)
+    bake.walk(u, settled_volume, walked)
+    marched = api.Targets(ctx, w, h)
+    ctx.march_image(u, marched, settled_volume, tables, api.INTEGRATOR_REFERENCE)
+    a = walked.read()[0].astype(np.float64)
+    b = marched.read()[0].astype(np.float64)
+    la = 0.2126 * a[..., 0] + 0.7152 * a[..., 1] + 0.0722 * a[..., 2]
+    lb = 0.2126 * b[..., 0] + 0.7152 * b[..., 1] + 0.0722 * b[..., 2]
+    diff = np.abs(la - lb).sum() / lb.sum()
+    headers, _ = bake.read()
+    print(f"walk vs march {diff:.3e}, overflow {bake.overflow()}, mean samples {headers[:, 0].mean():.1f}")
+    assert diff < 0.05 and headers[:, 0].max() <= 96
+    for o in (bake, walked, marched, tables):
+        o.close()
+
+
+def test_tonemap_matches_restatement(ctx):
+    vol = disk_volume(ctx)
+    req = kerr_request(ctx, 96, 54, 0.9, 60.0, 50.0, 1024, vol)
+    t = req["targets"]
+    ctx.march_image(req["uniforms"], t, vol, None, api.INTEGRATOR_FAST)
+    rgba = t.tonemap(exposure=1.0)
+    color = t.read()[0].astype(np.float64)
+    out = (C.c_double * 3)()
+    worst = 0
+    for y in range(0, 54, 3):
+        for x in range(0, 96, 3):
+            O.lib.orc_pbr_neutral(O.vec(np.maximum(color[y, x, :3], 0)), out)
+            want = [round(255 * O.lib.orc_srgb_encode(c)) for c in out]
+            worst = max(worst, max(abs(int(rgba[y, x, i]) - want[i]) for i in range(3)))
+    assert worst <= 1 and np.all(rgba[..., 3] == 255)

@@ -1,0 +1,152 @@
+"""GPU parity of the sim and splat passes: the determinism and isco goldens,
+one sim dispatch bit-exact against a CPU replay of the same float
+operations, the integer deposit bit-exact against a CPU replay (exact and
+stochastic corners), and the resolve against a float64 restatement."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gargantua_amd import api
+from tests.conftest import golden_passes, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def settled(ctx, count, seed, dispatches, generator=api.GENERATOR_DISK):
+    p = api.Particles(ctx, count, seed)
+    p.seed(generator)
+    if dispatches:
+        p.step(dispatches)
+    return p
+
+
+def test_determinism_golden(ctx):
+    g = load_golden("determinism")
+    steps = int(g["parameters"]["steps"]) // 2
+    sums = []
+    for _ in range(2):
+        p = settled(ctx, 250_000, 1, steps)
+        sums.append(p.checksum())
+        p.close()
+    print(f"{sums[0]:016x} vs {sums[1]:016x}")
+    assert golden_passes(g, float(sums[0] - sums[1]))
+
+
+def test_checksum_is_fnv1a_of_the_records(ctx):
+    p = settled(ctx, 10_000, 5, 3)
+    rec = p.read()
+    assert p.checksum() == O.lib.orc_fnv1a(rec.ctypes.data_as(C.c_void_p), rec.nbytes)
+    p.close()
+
+
+def test_isco_golden(ctx):
+    """1M particles, seed 1, 1500 substeps, edge from Sigma r^1.5 (Validate.swift:102-110)."""
+    g = load_golden("isco")
+    p = settled(ctx, 1_000_000, 1, int(g["parameters"]["settlingSteps"]) // 2)
+    rec = p.read()
+    radii = np.ascontiguousarray(np.hypot(rec["position"][:, 0], rec["position"][:, 1]).astype(np.float64))
+    q = g["parameters"]
+    edge = O.lib.orc_inner_edge(O.ptr(radii), len(radii), q["binWidth"], q["fitInner"], q["fitOuter"])
+    print(f"isco edge r = {edge:.3f}; respawns {int(rec['respawns'].sum())}")
+    assert golden_passes(g, edge) and rec["respawns"].sum() > 0
+    p.close()
+
+
+@pytest.mark.parametrize("generator", [api.GENERATOR_DISK, api.GENERATOR_ANNULUS])
+def test_sim_dispatch_bit_exact_against_cpu_replay(ctx, generator):
+    p = settled(ctx, 200_000, 7, 40, generator)
+    before = p.read()
+    p.step(1)
+    after = p.read()
+    replay = before.copy()
+    respawned = np.zeros(len(replay), np.uint8)
+    O.lib.orc_sim_replay_step(replay.ctypes.data_as(C.c_void_p), len(replay), 1, 0.1, 0.003, _temperature_scale(),
+                              respawned.ctypes.data_as(C.POINTER(C.c_uint8)))
+    keep = respawned == 0
+    assert keep.sum() > 0.9 * len(keep)
+    # Positions and velocities round identically; respawned particles differ.
+    assert np.array_equal(after["position"][keep].view(np.uint32), replay["position"][keep].view(np.uint32))
+    assert np.array_equal(after["velocity"][keep].view(np.uint32), replay["velocity"][keep].view(np.uint32))
+    assert np.array_equal(after["respawns"][keep], before["respawns"][keep])
+    assert np.all(after["respawns"][~keep] == before["respawns"][~keep] + 1)
+    # Temperature goes through pow (ocml vs libm): equal to a few ulp.
+    rel = np.abs(after["temperature"][keep] - replay["temperature"][keep]) / replay["temperature"][keep]
+    assert rel.max() < 4e-7
+    p.close()
+
+
+def _temperature_scale():
+    peak_r = 6.0 * 49.0 / 36.0
+    return 8000.0 / O.lib.orc_disk_shape(peak_r, 0.01)
+
+
+def test_seed_matches_oracle_spawn(ctx):
+    """Seeded particles against the FP64 spawn of the same hashes."""
+    p = settled(ctx, 4096, 11, 0)
+    rec = p.read()
+    cdf = O.inverse_cdf(1024)
+    s = (C.c_double * 6)()
+    worst = 0.0
+    for i in range(0, 4096, 7):
+        O.lib.orc_spawn(i, 0, 11, O.ptr(cdf), 1024, 1, 0, 0.0, s)
+        got = np.concatenate([rec["position"][i], rec["velocity"][i]]).astype(np.float64)
+        worst = max(worst, np.max(np.abs(got - np.array(s[:]))))
+    assert worst < 2e-4
+    p.close()
+
+
+@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("size,hz", [(96, 30.0), (128, 3.0)])
+def test_splat_bit_exact_against_cpu_replay(ctx, exact, size, hz):
+    p = settled(ctx, 250_000, 3, 100)
+    v = api.Volume(ctx, size, 30.0, hz)
+    v.splat(p, exact=exact, frame=17)
+    e, m, a = v.read_sums()
+    rec = p.read()
+    re = np.zeros_like(e)
+    rm = np.zeros_like(m)
+    ra = np.zeros_like(a)
+    O.lib.orc_splat_replay(rec.ctypes.data_as(C.c_void_p), len(rec), size, 30.0, hz, 8000.0, 0.0005,
+                           1 if exact else 0, 17, 3, re.ctypes.data_as(C.POINTER(C.c_uint32)),
+                           rm.ctypes.data_as(C.POINTER(C.c_int32)), ra.ctypes.data_as(C.POINTER(C.c_uint32)))
+    assert a.sum() > 0
+    assert np.array_equal(e, re) and np.array_equal(m, rm) and np.array_equal(a, ra)
+    v.close()
+    p.close()
+
+
+def test_resolve_against_float64_restatement(ctx):
+    p = settled(ctx, 250_000, 3, 100)
+    v = api.Volume(ctx, 96)
+    v.splat(p, exact=True)
+    e, m, a = v.read_sums()
+    v.resolve(clear=True)
+    emission, velocity = v.read_fields()
+    e2, m2, a2 = v.read_sums()
+    assert not e2.any() and not m2.any() and not a2.any()
+    voxel = 60.0 / 96
+    count = a.astype(np.float64) / 256.0
+    energy = e[0::2].astype(np.float64) / 2 ** 24
+    energy_t = e[1::2].astype(np.float64) / 2 ** 20
+    density = count / voxel ** 3
+    table = O.blackbody_table()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        mean_t = np.where(energy > 0, energy_t / energy * 8000.0, 1000.0)
+    coord = (np.log(np.clip(mean_t, 1000, 40000)) - np.log(1000)) / (np.log(40000) - np.log(1000))
+    x = coord * 256 - 0.5
+    i0 = np.clip(np.floor(x).astype(int), 0, 255)
+    i1 = np.clip(np.floor(x).astype(int) + 1, 0, 255)
+    f = (x - np.floor(x))[:, None]
+    tint = table[i0] * (1 - f) + table[i1] * f
+    want_e = tint * (energy / voxel ** 3)[:, None]
+    got = emission.reshape(-1, 4)
+    ok = density > 0
+    assert np.allclose(got[:, 3], density, rtol=2e-3, atol=1e-6)
+    assert np.allclose(got[ok, :3], want_e[ok], rtol=4e-3, atol=1e-6)
+    mom = m.reshape(-1, 3).astype(np.float64) / 4096.0
+    want_v = np.where(count[:, None] > 0, mom / np.maximum(count, 1e-30)[:, None], 0.0)
+    assert np.allclose(velocity.reshape(-1, 4)[:, :3], want_v, rtol=2e-3, atol=2e-3)
+    v.close()
+    p.close()
