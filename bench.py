@@ -156,6 +156,17 @@ def main():
         plain = {"ms_per_step": plain_ms_per, "value": plain_steps / (plain_ms_per * 1e-3),
                  "steps_per_frame": plain_steps, "integrator": "reference", "launch": "one per frame, 8x8 waves",
                  "p50_ms": float(np.percentile(plain_frames, 50)), "p99_ms": float(np.percentile(plain_frames, 99))}
+        # Decomposition: this library's integrator with the plain launch, so
+        # the gain splits into arithmetic (plain_fast / plain) and scheduling
+        # (batched / plain_fast).
+        plain_targets.reset_counters()
+        ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_FAST)
+        fast_steps = int(plain_targets.counters().steps)
+        fast_ms, _ = time_frames(ctx, api, lambda: ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_FAST),
+                                 max(1, args.warmup // 2), max(3, args.steps // 2))
+        fast_ms_per = fast_ms / max(3, args.steps // 2)
+        plain["fast_integrator"] = {"ms_per_step": fast_ms_per, "value": fast_steps / (fast_ms_per * 1e-3),
+                                    "steps_per_frame": fast_steps}
         plain_targets.close()
 
     achieved = steps_per_frame * FLOPS_PER_KERR_STEP / (ms_per_step * 1e-3) / 1e12
@@ -189,7 +200,7 @@ def main():
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "compute_units": info.compute_units},
     }
     if not args.no_cpu_baseline:
-        every = args.cpu_rows or 4
+        every = args.cpu_rows or 1
         line["cpu_baseline"] = cpu_baseline(u, vol, every)
     else:
         line["cpu_baseline"] = None

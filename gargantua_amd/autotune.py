@@ -1,0 +1,112 @@
+"""The batch autotuner: sweeps block size, resident waves per SIMD,
+compaction period and refill threshold of the persistent march on a
+workload, and persists the fastest configuration per (arch, workload) in
+gargantua_amd/autotune.json, which callers load with `best()`.
+
+python -m gargantua_amd.autotune --workload headline|mixed
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import time
+
+from . import api, workloads
+
+PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
+
+GRID = {
+    "block": [64, 128, 256],
+    "waves_per_simd": [2, 4, 8],
+    "compaction_period": [8, 16, 32, 64, 128],
+    "refill_threshold": [16, 32, 48, 63],
+}
+
+
+def time_config(ctx: api.Context, requests: list, cfg: api.BatchConfig, repeats: int) -> float:
+    ctx.march_batch(requests, cfg)  # warm
+    a, b = ctx.event(), ctx.event()
+    a.record()
+    for _ in range(repeats):
+        _enqueue(ctx, requests, cfg)
+    b.record()
+    b.sync()
+    return a.elapsed_ms(b) / repeats
+
+
+def _enqueue(ctx, requests, cfg):
+    import ctypes as C
+    arr = (api.MarchRequest * len(requests))()
+    for i, r in enumerate(requests):
+        arr[i].uniforms = r["uniforms"]
+        arr[i].targets = r["targets"].h
+        arr[i].tables = r["tables"].h if r.get("tables") else None
+        arr[i].volume = r["volume"].h if r.get("volume") else None
+    st = api.load().garg_march_batch(ctx.h, None, arr, len(requests), C.byref(cfg), None)
+    if st != 0:
+        raise api.GargError(st, api.load().garg_last_error().decode())
+
+
+def sweep(ctx: api.Context, requests: list, grid: dict | None = None, repeats: int = 2, log=print) -> list:
+    grid = grid or GRID
+    results = []
+    for block, waves, period, thr in itertools.product(*(grid[k] for k in GRID)):
+        cfg = ctx.batch_config(block=block, waves_per_simd=waves, compaction_period=period, refill_threshold=thr)
+        ms = time_config(ctx, requests, cfg, repeats)
+        results.append(({"block": block, "waves_per_simd": waves, "compaction_period": period,
+                          "refill_threshold": thr}, ms))
+        log(f"block {block:3d} waves {waves} period {period:3d} threshold {thr:2d}: {ms:9.3f} ms")
+    results.sort(key=lambda r: r[1])
+    return results
+
+
+def best(ctx: api.Context, workload: str) -> api.BatchConfig:
+    """The persisted winner for this device and workload, else the default."""
+    arch = ctx.info().arch.decode()
+    try:
+        table = json.load(open(PATH))
+        entry = table.get(arch, {}).get(workload)
+        if entry:
+            return ctx.batch_config(**entry["config"])
+    except (OSError, ValueError):
+        pass
+    return ctx.batch_config()
+
+
+def save(arch: str, workload: str, config: dict, ms: float):
+    try:
+        table = json.load(open(PATH))
+    except (OSError, ValueError):
+        table = {}
+    table.setdefault(arch, {})[workload] = {"config": config, "ms": ms, "date": time.strftime("%Y-%m-%d")}
+    with open(PATH, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description="persistent march autotuner")
+    ap.add_argument("--workload", choices=["headline", "mixed"], default="headline")
+    ap.add_argument("--repeats", type=int, default=2)
+    ap.add_argument("--quick", action="store_true", help="a reduced grid")
+    args = ap.parse_args()
+    ctx = api.Context()
+    p, v = workloads.disk(ctx)
+    reqs = [workloads.headline(ctx, v)] if args.workload == "headline" else workloads.mixed(ctx, v)
+    grid = GRID
+    if args.quick:
+        grid = {"block": [64, 256], "waves_per_simd": [4, 8], "compaction_period": [16, 64],
+                "refill_threshold": [32, 63]}
+    results = sweep(ctx, reqs, grid, args.repeats)
+    cfg, ms = results[0]
+    save(ctx.info().arch.decode(), args.workload, cfg, ms)
+    print(json.dumps({"workload": args.workload, "best": cfg, "ms": ms, "worst_ms": results[-1][1]}))
+    workloads.close(reqs)
+    v.close()
+    p.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "fast_math.hpp"
 
 namespace garg {
 
@@ -571,10 +572,29 @@ struct KerrEval {
 };
 
 GARG_DEV float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+GARG_DEV float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+GARG_DEV f3 kerr_position_fast(float a2, float r, float st, float ct, float phi) {
+    float sp, cp;
+    fast_sincos(phi, &sp, &cp);
+    float radial = fast_sqrt(r * r + a2);
+    return f3{radial * st * cp, radial * st * sp, r * ct};
+}
+
+GARG_DEV f3 kerr_direction_fast(float r, float st, float ct, float phi, float rdot, float thdot, float phdot) {
+    float sp, cp;
+    fast_sincos(phi, &sp, &cp);
+    f3 r_hat{st * cp, st * sp, ct};
+    f3 theta_hat{ct * cp, ct * sp, -st};
+    f3 phi_hat{-sp, cp, 0.0f};
+    f3 local = r_hat * rdot + theta_hat * (r * thdot) + phi_hat * (r * st * phdot);
+    float len2 = dot(local, local);
+    return len2 > 0.0f ? local * __builtin_amdgcn_rsqf(len2) : r_hat;
+}
 
 GARG_DEV KerrEval kerr_eval(const KerrConsts& k, float r, float th, float pr, float pth) {
     KerrEval e;
-    sincosf(th, &e.st, &e.ct);
+    fast_sincos(th, &e.st, &e.ct);
     float r2 = r * r;
     float delta = (r2 - 2.0f * r) + k.a2;
     float sigma = r2 + k.a2 * (e.ct * e.ct);
@@ -609,7 +629,7 @@ struct KerrFastMarch {
     float r, th, ph, pr, pth;
     float cr, cth, cph, cpr, cpth;  // TwoSum carries
     KerrEval e;                      // evaluation at the current state
-    float carter0, scale, capture;
+    float carter0, inv_scale, capture;
     Gather g;
     f3 prev;
     float drift;
@@ -627,10 +647,10 @@ struct KerrFastMarch {
         cr = cth = cph = cpr = cpth = 0.0f;
         e = kerr_eval(k, r, th, pr, pth);
         carter0 = e.carter;
-        scale = fmaxf(fabsf(carter0), k.E * k.E);
+        inv_scale = 1.0f / fmaxf(fabsf(carter0), k.E * k.E);
         capture = k.rplus + GARG_F(KERR_CAPTURE_MARGIN);
         g.reset();
-        prev = kerr_position_sc(k.a, r, e.st, e.ct, ph);
+        prev = kerr_position_fast(k.a2, r, e.st, e.ct, ph);
         drift = 0.0f;
         steps = 0;
         outcome = GARG_RAY_EXHAUSTED;
@@ -646,13 +666,13 @@ struct KerrFastMarch {
                 break;
             }
             float near_h = 0.2f * (r - k.rplus);
-            float pole = fminf(fmaxf(e.st / GARG_F(KERR_POLE_SINE), 0.05f), 1.0f);
+            float pole = fminf(fmaxf(e.st * (1.0f / GARG_F(KERR_POLE_SINE)), 0.05f), 1.0f);
             float h = fminf(fmaxf(fminf(gs.step_factor * r, near_h) * pole, GARG_F(KERR_STEP_FLOOR)), gs.step_max);
             float hh = 0.5f * h;
             KerrEval k2 = kerr_eval(k, r + e.dr * hh, th + e.dth * hh, pr + e.dpr * hh, pth + e.dpth * hh);
             KerrEval k3 = kerr_eval(k, r + k2.dr * hh, th + k2.dth * hh, pr + k2.dpr * hh, pth + k2.dpth * hh);
             KerrEval k4 = kerr_eval(k, r + k3.dr * h, th + k3.dth * h, pr + k3.dpr * h, pth + k3.dpth * h);
-            float w = h / 6.0f;
+            float w = h * (1.0f / 6.0f);
             float ir = ((e.dr + k2.dr * 2.0f) + (k3.dr * 2.0f + k4.dr)) * w;
             float ith = ((e.dth + k2.dth * 2.0f) + (k3.dth * 2.0f + k4.dth)) * w;
             float iph = ((e.dph + k2.dph * 2.0f) + (k3.dph * 2.0f + k4.dph)) * w;
@@ -673,11 +693,12 @@ struct KerrFastMarch {
             }
             steps += 1;
             e = kerr_eval(k, r, th, pr, pth);
-            f3 p = kerr_position_sc(k.a, r, e.st, e.ct, ph);
-            float path = length(p - prev);
+            f3 p = kerr_position_fast(k.a2, r, e.st, e.ct, ph);
+            f3 dp = p - prev;
+            float path = fast_sqrt(dot(dp, dp));
             g.travelled += path;
-            float carter_drift = fabsf(e.carter - carter0) / scale;
-            float null_drift = fabsf(e.null_residual) / (scale * fmaxf(r * r, 1.0f));
+            float carter_drift = fabsf(e.carter - carter0) * inv_scale;
+            float null_drift = fabsf(e.null_residual) * inv_scale * fast_rcp(fmaxf(r * r, 1.0f));
             drift = fmaxf(drift, fmaxf(carter_drift, null_drift));
             if (r <= capture) {
                 outcome = GARG_RAY_CAPTURED;
@@ -689,7 +710,7 @@ struct KerrFastMarch {
                 const KerrEval& ee = e;
                 float rr = r, pp = ph;
                 g.sample(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, fmaxf(f, 0.0f),
-                         [&]() { return kerr_direction_from(rr, ee.st, ee.ct, pp, ee.dr, ee.dth, ee.dph); });
+                         [&]() { return kerr_direction_fast(rr, ee.st, ee.ct, pp, ee.dr, ee.dth, ee.dph); });
             }
             prev = p;
             if (r >= gs.escape_radius) {
@@ -704,8 +725,8 @@ struct KerrFastMarch {
     GARG_DEV bool falls_in() const {
         return outcome == GARG_RAY_CAPTURED || (outcome == GARG_RAY_EXHAUSTED && pr < 0.0f);
     }
-    GARG_DEV f3 final_direction() const { return kerr_direction_from(r, e.st, e.ct, ph, e.dr, e.dth, e.dph); }
-    GARG_DEV f3 final_position() const { return kerr_position_sc(k.a, r, e.st, e.ct, ph); }
+    GARG_DEV f3 final_direction() const { return kerr_direction_fast(r, e.st, e.ct, ph, e.dr, e.dth, e.dph); }
+    GARG_DEV f3 final_position() const { return kerr_position_fast(k.a2, r, e.st, e.ct, ph); }
 };
 
 // Colour of a finished ray: gas plus the starfield behind it unless it falls

@@ -1,0 +1,64 @@
+// Sine and cosine for the fast Kerr integrator, written from the
+// mathematics: Cody and Waite's additive range reduction by multiples of
+// pi/2 (W. J. Cody and W. Waite, Software Manual for the Elementary
+// Functions, 1980, ch. 8) with the reduction done in fused multiply-adds,
+// then the Taylor series of sin and cos on [-pi/4, pi/4] truncated where the
+// next term is below 2^-28 (the sin series to r^9, the cos series to r^10),
+// evaluated by Horner's rule. Every operation is an explicit fmaf or a single
+// rounded operation, so host and device builds round identically, which the
+// CPU test of this function relies on.
+//
+// Valid for |x| < 2^12 (the march's theta stays near [0, pi] and its phi
+// within a few hundred radians); the error is a few ulp there.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define GARG_HD __host__ __device__ __forceinline__
+#else
+#define GARG_HD static inline
+#endif
+
+namespace garg {
+
+// pi/2 as a sum of three floats, each the float nearest to what the
+// previous ones leave (A + B + C = pi/2 to about 2^-74). Each reduction step
+// is one fused multiply-add, so k * A, k * B and k * C enter exactly.
+#define GARG_PIO2_A 1.5707963705062866f
+#define GARG_PIO2_B -4.3711388286737929e-08f
+#define GARG_PIO2_C -1.7151245100058819e-15f
+#define GARG_TWO_OVER_PI 0.636619772367581343f
+
+GARG_HD void sincos_reduced(float r, float* s, float* c) {
+    float r2 = r * r;
+    // sin r = r (1 - r^2/3! + r^4/5! - r^6/7! + r^8/9!)
+    float ps = fmaf(r2, 2.75573192239858907e-06f, -1.98412698412698413e-04f);
+    ps = fmaf(r2, ps, 8.33333333333333333e-03f);
+    ps = fmaf(r2, ps, -1.66666666666666667e-01f);
+    ps = ps * r2;
+    *s = fmaf(r, ps, r);
+    // cos r = 1 - r^2/2! + r^4/4! - r^6/6! + r^8/8! - r^10/10!
+    float pc = fmaf(r2, -2.75573192239858907e-07f, 2.48015873015873016e-05f);
+    pc = fmaf(r2, pc, -1.38888888888888889e-03f);
+    pc = fmaf(r2, pc, 4.16666666666666667e-02f);
+    pc = fmaf(r2, pc, -0.5f);
+    *c = fmaf(r2, pc, 1.0f);
+}
+
+GARG_HD void fast_sincos(float x, float* s, float* c) {
+    float k = rintf(x * GARG_TWO_OVER_PI);
+    float r = fmaf(-k, GARG_PIO2_A, x);
+    r = fmaf(-k, GARG_PIO2_B, r);
+    r = fmaf(-k, GARG_PIO2_C, r);
+    float sr, cr;
+    sincos_reduced(r, &sr, &cr);
+    int q = (int)k & 3;
+    float ss = (q & 1) ? cr : sr;
+    float cc = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -ss : ss;
+    *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+}  // namespace garg

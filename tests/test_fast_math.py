@@ -1,0 +1,58 @@
+"""The fast integrator's sincos (gargantua_amd/csrc/fast_math.hpp) against
+double-precision sin and cos on the CPU. The function uses only fmaf and
+single-rounded operations, so this host build rounds exactly like gfx950."""
+import os
+import subprocess
+import tempfile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+HARNESS = r"""
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include "fast_math.hpp"
+static double ulp_err(float got, double want) {
+    float w = (float)want;
+    float next = nextafterf(fabsf(w), INFINITY);
+    double ulp = (double)next - (double)fabsf(w);
+    if (w == 0.0f) ulp = 1.4e-45;
+    return fabs((double)got - want) / ulp;
+}
+int main(void) {
+    double worst_s = 0, worst_c = 0, worst_abs = 0, worst_x = 0;
+    for (long i = -4000000; i <= 4000000; ++i) {
+        float x = (float)(i * 1e-5);   /* [-40, 40] */
+        float s, c;
+        garg::fast_sincos(x, &s, &c);
+        double ws = sin((double)x), wc = cos((double)x);
+        double es = ulp_err(s, ws), ec = ulp_err(c, wc);
+        double ea = fmax(fabs(s - ws), fabs(c - wc));
+        if (fabs(ws) > 1e-3 && es > worst_s) { worst_s = es; worst_x = x; }
+        if (fabs(wc) > 1e-3 && ec > worst_c) worst_c = ec;
+        if (ea > worst_abs) worst_abs = ea;
+    }
+    /* small angles near the pole keep relative accuracy */
+    double worst_small = 0;
+    for (int k = 1; k < 100000; ++k) {
+        float x = (float)k * 1e-7f;
+        float s, c;
+        garg::fast_sincos(x, &s, &c);
+        worst_small = fmax(worst_small, fabs(s - sin((double)x)) / sin((double)x));
+    }
+    printf("%g %g %g %g %g\n", worst_s, worst_c, worst_abs, worst_small, worst_x);
+    return 0;
+}
+"""
+
+
+def test_fast_sincos_accuracy():
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "h.cpp"), os.path.join(d, "h")
+        open(src, "w").write(HARNESS)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+                        os.path.join(ROOT, "gargantua_amd", "csrc"), src, "-o", exe, "-lm"], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    worst_s, worst_c, worst_abs, worst_small = (float(v) for v in out[:4])
+    print(f"sin {worst_s:.2f} ulp, cos {worst_c:.2f} ulp, abs {worst_abs:.2e}, small-angle rel {worst_small:.2e}")
+    assert worst_s < 4 and worst_c < 4 and worst_abs < 3e-7 and worst_small < 2e-7
