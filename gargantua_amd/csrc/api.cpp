@@ -157,7 +157,9 @@ int sync_stream(hipStream_t s, const char* what) {
 
 Scene make_scene(garg_ctx* ctx, garg_volume* volume, garg_tables* tables) {
     Scene sc{};
-    if (volume) sc.volume = VolumeView{volume->voxels, (int)volume->size, volume->half_xy, volume->half_z};
+    if (volume)
+        sc.volume = VolumeView{volume->voxels, (int)volume->size, volume->half_xy, volume->half_z,
+                               (float)volume->size / (2.0f * volume->half_xy), (float)volume->size / (2.0f * volume->half_z)};
     if (tables) {
         sc.sphere_table = tables->data;
         sc.camera_table = tables->data + SWEEP_TABLE_SIZE;

@@ -68,6 +68,8 @@ struct VolumeView {
     int size;
     float half_xy;
     float half_z;
+    float scale_xy;  // size / (2 half_xy): texels per unit length
+    float scale_z;   // size / (2 half_z)
 };
 
 struct Sample {
@@ -80,10 +82,9 @@ struct Sample {
 // semantics of the reference's clamp_to_zero sampler (march.metal:15-27) as
 // the oracle states them (VolumeField.swift:40-63).
 GARG_DEV Sample sample_volume(const VolumeView& v, f3 p) {
-    const float n = (float)v.size;
-    float tx = (p.x + v.half_xy) / (2.0f * v.half_xy) * n - 0.5f;
-    float ty = (p.y + v.half_xy) / (2.0f * v.half_xy) * n - 0.5f;
-    float tz = (p.z + v.half_z) / (2.0f * v.half_z) * n - 0.5f;
+    float tx = (p.x + v.half_xy) * v.scale_xy - 0.5f;
+    float ty = (p.y + v.half_xy) * v.scale_xy - 0.5f;
+    float tz = (p.z + v.half_z) * v.scale_z - 0.5f;
     float bx = floorf(tx), by = floorf(ty), bz = floorf(tz);
     float fx = tx - bx, fy = ty - by, fz = tz - bz;
     int x0 = (int)bx, y0 = (int)by, z0 = (int)bz;

@@ -600,9 +600,14 @@ GARG_DEV KerrEval kerr_eval(const KerrConsts& k, float r, float th, float pr, fl
     float sigma = r2 + k.a2 * (e.ct * e.ct);
     float inv_sigma = fast_rcp(sigma);
     float inv_delta = fast_rcp(delta);
-    float sin2 = fmaxf(e.st * e.st, 1e-8f);
-    float inv_sin2 = fast_rcp(sin2);
-    float inv_ms = fast_rcp(fmaxf(e.st, 1e-4f));
+    // max(sin^2, 1e-8) = max(|sin|, 1e-4)^2: one reciprocal serves both
+    // clamped terms; max(sin, 1e-4) differs from max(|sin|, 1e-4) only past
+    // the pole, where sin < 0.
+    float m = fmaxf(fabsf(e.st), 1e-4f);
+    float sin2 = m * m;
+    float inv_m = fast_rcp(m);
+    float inv_sin2 = inv_m * inv_m;
+    float inv_ms = e.st >= 1e-4f ? inv_m : 1e4f;
     float P = (r2 + k.a2) * k.E - k.aL;
     float P_over_delta = P * inv_delta;
     float G = (k.L2 * inv_sin2 - k.two_aEL) + k.a2E2 * sin2;
@@ -678,19 +683,14 @@ struct KerrFastMarch {
             float iph = ((e.dph + k2.dph * 2.0f) + (k3.dph * 2.0f + k4.dph)) * w;
             float ipr = ((e.dpr + k2.dpr * 2.0f) + (k3.dpr * 2.0f + k4.dpr)) * w;
             float ipth = ((e.dpth + k2.dpth * 2.0f) + (k3.dpth * 2.0f + k4.dpth)) * w;
-            if (compensated) {
-                two_sum(r, cr, ir);
-                two_sum(th, cth, ith);
-                two_sum(ph, cph, iph);
-                two_sum(pr, cpr, ipr);
-                two_sum(pth, cpth, ipth);
-            } else {
-                r += ir;
-                th += ith;
-                ph += iph;
-                pr += ipr;
-                pth += ipth;
-            }
+            // TwoSum always; without compensation the carries stay zero, which
+            // makes each update the plain s + increment.
+            two_sum(r, cr, ir);
+            two_sum(th, cth, ith);
+            two_sum(ph, cph, iph);
+            two_sum(pr, cpr, ipr);
+            two_sum(pth, cpth, ipth);
+            if (!compensated) cr = cth = cph = cpr = cpth = 0.0f;
             steps += 1;
             e = kerr_eval(k, r, th, pr, pth);
             f3 p = kerr_position_fast(k.a2, r, e.st, e.ct, ph);
