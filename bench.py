@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=0, help="march every k-th row on the CPU (0 = auto)")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--config", default="", help="batch config as block,waves,period,threshold")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 64-request mixed batch")
+    ap.add_argument("--no-passes", action="store_true", help="skip the sim/splat/resolve and FP64 legs")
     return ap.parse_args()
 
 
@@ -126,7 +128,8 @@ def main():
     info = ctx.info()
     p, vol, u, targets = build(ctx, api)
 
-    cfg = ctx.batch_config()
+    from gargantua_amd import autotune
+    cfg = autotune.best(ctx, "headline")
     if args.config:
         b, w, per, thr = (int(x) for x in args.config.split(","))
         cfg = ctx.batch_config(block=b, waves_per_simd=w, compaction_period=per, refill_threshold=thr)
@@ -196,9 +199,16 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "peak_nonpacked": FP32_PEAK_NONPACKED_TFLOPS,
-                     "flops_per_step": FLOPS_PER_KERR_STEP, "kernel": "march_batch_kernel<true,256>"},
+                     "flops_per_step": FLOPS_PER_KERR_STEP,
+                     "kernel": ("march_tiles_kernel" if cfg.refill_threshold == 0 else "march_batch_kernel")
+                     + f"<true,{cfg.block}>"},
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "compute_units": info.compute_units},
     }
+    if not args.no_mixed:
+        line["mixed"] = mixed_leg(ctx, api, vol, args)
+    if not args.no_passes:
+        line["passes"] = pass_leg(ctx, api, p, vol)
+        line["fp64"] = fp64_leg(ctx)
     if not args.no_cpu_baseline:
         every = args.cpu_rows or 1
         line["cpu_baseline"] = cpu_baseline(u, vol, every)
@@ -210,6 +220,85 @@ def main():
     ctx.close()
 
 
+def mixed_leg(ctx, api, vol, args):
+    """64 requests (resolution, spin, inclination, observer radius and
+    budget drawn from seed 4) in one batched launch against the same
+    requests marched one plain launch each, both integrator forms."""
+    from gargantua_amd import autotune, workloads
+    reqs = workloads.mixed(ctx, vol)
+    cfg = autotune.best(ctx, "mixed")
+    for r in reqs:
+        r["targets"].reset_counters()
+    totals = ctx.march_batch(reqs, cfg)
+    steps = int(totals.steps)
+    batched_ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, reqs, cfg), 1, 3)
+    out = {"requests": len(reqs), "rays": int(totals.rays), "steps": steps, "captured": int(totals.captured),
+           "exhausted": int(totals.exhausted),
+           "batch": {"block": cfg.block, "waves_per_simd": cfg.waves_per_simd,
+                     "compaction_period": cfg.compaction_period, "refill_threshold": cfg.refill_threshold},
+           "batched_ms": batched_ms / 3, "batched_steps_per_s": steps / (batched_ms / 3 * 1e-3)}
+    for name, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
+        def run_all():
+            for r in reqs:
+                ctx.march_image(r["uniforms"], r["targets"], vol, None, integrator)
+        ms, _ = time_frames(ctx, api, run_all, 1, 2)
+        out[name + "_ms"] = ms / 2
+        out[name + "_steps_per_s"] = steps / (ms / 2 * 1e-3)
+    out["speedup_vs_plain_reference"] = out["plain_reference_ms"] / out["batched_ms"]
+    out["speedup_vs_plain_fast"] = out["plain_fast_ms"] / out["batched_ms"]
+    workloads.close(reqs)
+    return out
+
+
+def pass_leg(ctx, api, particles, vol):
+    """Disk passes at 1M particles into 128^3: one sim dispatch (two
+    substeps), the exact splat bin and the resolve with its fused clear,
+    against their algorithmic bytes."""
+    n = 1_000_000
+    voxels = 128 ** 3
+
+    def timed(fn, reps):
+        fn()
+        ctx.sync()
+        a, b = ctx.event(), ctx.event()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.sync()
+        return a.elapsed_ms(b) / reps
+
+    sim_ms = timed(lambda: particles.step(1), 50)
+    bin_ms = timed(lambda: vol.splat(particles, exact=True), 10)
+    resolve_ms = timed(lambda: (vol.splat(particles, exact=True), vol.resolve(clear=True)), 10) - bin_ms
+    sim_bytes = 64 * n
+    resolve_bytes = (24 + 16 + 24) * voxels
+    return {"sim": {"ms": sim_ms, "particles_per_s": n / (sim_ms * 1e-3), "bytes": sim_bytes,
+                    "GB_per_s": sim_bytes / (sim_ms * 1e-3) / 1e9, "hbm_frac": sim_bytes / (sim_ms * 1e-3) / 8e12},
+            "splat_bin_exact": {"ms": bin_ms, "deposits_per_s": n / (bin_ms * 1e-3),
+                                "corner_atomics_per_s": 8 * 6 * n / (bin_ms * 1e-3)},
+            "resolve": {"ms": resolve_ms, "voxels_per_s": voxels / (resolve_ms * 1e-3), "bytes": resolve_bytes,
+                        "GB_per_s": resolve_bytes / (resolve_ms * 1e-3) / 1e9,
+                        "hbm_frac": resolve_bytes / (resolve_ms * 1e-3) / 8e12}}
+
+
+def fp64_leg(ctx):
+    """FP64 GPU kernels of the spin 0.9 view: the capture boundary along 360
+    screen azimuths (multi-section) and the 4096-entry far-field table on
+    log-spaced b in [5.2, 1000]."""
+    t0 = time.perf_counter()
+    edge, _ = ctx.shadow_boundary(0.9, 50.0, 60.0, 360)
+    shadow_ms = (time.perf_counter() - t0) * 1e3
+    phi = np.arange(360) * (2 * math.pi / 360)
+    x = edge * np.cos(phi)
+    t0 = time.perf_counter()
+    ctx.deflection_table(50.0, 4096, 5.2, 1000.0)
+    table_ms = (time.perf_counter() - t0) * 1e3
+    return {"shadow_boundary": {"azimuths": 360, "ms": shadow_ms, "width_rad": float(x.max() - x.min()),
+                                "centre_offset_rad": float(0.5 * (x.max() + x.min()))},
+            "far_field_table": {"entries": 4096, "ms": table_ms}}
+
+
 def _batch_nosync(ctx, api, request, cfg):
     """One frame of the batched march, enqueued without reading counters back."""
     import ctypes as C
@@ -218,6 +307,8 @@ def _batch_nosync(ctx, api, request, cfg):
         arr[i].uniforms = r["uniforms"]
         arr[i].targets = r["targets"].h
         arr[i].volume = r["volume"].h if r.get("volume") else None
+        arr[i].ray_begin = r.get("ray_begin", 0)
+        arr[i].ray_count = r.get("ray_count", 0)
     st = api.load().garg_march_batch(ctx.h, None, arr, len(request), C.byref(cfg), None)
     if st != 0:
         raise api.GargError(st, api.load().garg_last_error().decode())

@@ -17,11 +17,13 @@ from . import api, workloads
 
 PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
 
+# refill_threshold 0 selects whole-tile claims, where the compaction period
+# plays no part; the grid visits it once per block size and wave count.
 GRID = {
     "block": [64, 128, 256],
-    "waves_per_simd": [2, 4, 8],
-    "compaction_period": [8, 16, 32, 64, 128],
-    "refill_threshold": [16, 32, 48, 63],
+    "waves_per_simd": [4, 5, 8],
+    "compaction_period": [16, 32, 64, 128],
+    "refill_threshold": [0, 16, 32, 48, 63],
 }
 
 
@@ -52,7 +54,12 @@ def _enqueue(ctx, requests, cfg):
 def sweep(ctx: api.Context, requests: list, grid: dict | None = None, repeats: int = 2, log=print) -> list:
     grid = grid or GRID
     results = []
+    seen = set()
     for block, waves, period, thr in itertools.product(*(grid[k] for k in GRID)):
+        if thr == 0:
+            if (block, waves) in seen:
+                continue
+            seen.add((block, waves))
         cfg = ctx.batch_config(block=block, waves_per_simd=waves, compaction_period=period, refill_threshold=thr)
         ms = time_config(ctx, requests, cfg, repeats)
         results.append(({"block": block, "waves_per_simd": waves, "compaction_period": period,
@@ -96,8 +103,8 @@ def main():
     reqs = [workloads.headline(ctx, v)] if args.workload == "headline" else workloads.mixed(ctx, v)
     grid = GRID
     if args.quick:
-        grid = {"block": [64, 256], "waves_per_simd": [4, 8], "compaction_period": [16, 64],
-                "refill_threshold": [32, 63]}
+        grid = {"block": [64, 256], "waves_per_simd": [4, 5], "compaction_period": [16, 64],
+                "refill_threshold": [0, 32, 63]}
     results = sweep(ctx, reqs, grid, args.repeats)
     cfg, ms = results[0]
     save(ctx.info().arch.decode(), args.workload, cfg, ms)

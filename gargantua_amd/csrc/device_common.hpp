@@ -88,6 +88,9 @@ GARG_DEV Sample sample_volume(const VolumeView& v, f3 p) {
     float bx = floorf(tx), by = floorf(ty), bz = floorf(tz);
     float fx = tx - bx, fy = ty - by, fz = tz - bz;
     int x0 = (int)bx, y0 = (int)by, z0 = (int)bz;
+    // The voxels live in global memory: saying so turns generic (flat) loads,
+    // which also wait on the LDS counter, into global ones.
+    const __attribute__((address_space(1))) Voxel* g = (const __attribute__((address_space(1))) Voxel*)v.voxels;
     Sample s{f3{0, 0, 0}, 0.0f, f3{0, 0, 0}};
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -95,10 +98,12 @@ GARG_DEV Sample sample_volume(const VolumeView& v, f3 p) {
         float w = ((c & 1) ? fx : 1.0f - fx) * (((c >> 1) & 1) ? fy : 1.0f - fy) * (((c >> 2) & 1) ? fz : 1.0f - fz);
         bool inside = (unsigned)x < (unsigned)v.size && (unsigned)y < (unsigned)v.size && (unsigned)z < (unsigned)v.size;
         if (inside && w != 0.0f) {
-            const Voxel t = v.voxels[((int64_t)z * v.size + y) * v.size + x];
-            s.emission = s.emission + f3{(float)t.emission.x, (float)t.emission.y, (float)t.emission.z} * w;
-            s.density += (float)t.emission.w * w;
-            s.velocity = s.velocity + f3{(float)t.velocity.x, (float)t.velocity.y, (float)t.velocity.z} * w;
+            uint32_t i = ((uint32_t)z * (uint32_t)v.size + (uint32_t)y) * (uint32_t)v.size + (uint32_t)x;
+            const h4 em = g[i].emission;
+            const h4 ve = g[i].velocity;
+            s.emission = s.emission + f3{(float)em.x, (float)em.y, (float)em.z} * w;
+            s.density += (float)em.w * w;
+            s.velocity = s.velocity + f3{(float)ve.x, (float)ve.y, (float)ve.z} * w;
         }
     }
     return s;
@@ -234,6 +239,20 @@ struct Scene {
     const float* sky_table;
     const float4* blackbody;    // BLACKBODY_TABLE_SIZE entries, rgb + 1
 };
+
+// A copy of `v` with every 32-bit word read from the first active lane. For
+// data the whole wave shares this keeps it in scalar registers: the loop
+// that uses it then reads SGPR operands instead of reloading per-lane copies.
+template <class T>
+GARG_DEV T wave_uniform(const T& v) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized copy");
+    T out;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&v);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return out;
+}
 
 // Sum of a value over the active lanes of a wave64.
 GARG_DEV uint64_t wave_sum_u64(uint64_t v) {

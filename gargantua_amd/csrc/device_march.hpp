@@ -629,6 +629,32 @@ GARG_DEV KerrEval kerr_eval(const KerrConsts& k, float r, float th, float pr, fl
     return e;
 }
 
+// What the fast Kerr loop reads besides the ray: a by-value block, so a
+// caller whose wave shares one request can hold it in scalar registers.
+struct KerrLoop {
+    uint32_t step_cap;
+    float step_factor, step_max, escape_radius;
+    float opacity;
+    uint32_t redshift;
+    uint32_t compensated;
+    uint32_t gather;  // sample the volume (non-zero only when there is one)
+    VolumeView volume;
+};
+
+GARG_DEV KerrLoop kerr_loop(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated) {
+    KerrLoop l;
+    l.step_cap = u.geodesic.step_cap;
+    l.step_factor = u.geodesic.step_factor;
+    l.step_max = u.geodesic.step_max;
+    l.escape_radius = u.geodesic.escape_radius;
+    l.opacity = u.opacity_scale;
+    l.redshift = u.redshift;
+    l.compensated = compensated ? 1u : 0u;
+    l.gather = gather && sc.volume.voxels != nullptr ? 1u : 0u;
+    l.volume = sc.volume;
+    return l;
+}
+
 struct KerrFastMarch {
     KerrConsts k;
     float r, th, ph, pr, pth;
@@ -663,16 +689,24 @@ struct KerrFastMarch {
     }
 
     GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget) {
-        const bool has_volume = gather && sc.volume.voxels != nullptr;
-        const garg_geodesic_settings& gs = u.geodesic;
+        run_loop<false>(kerr_loop(u, sc, gather, compensated), budget);
+    }
+
+    // Advance by at most `budget` steps. UNIFORM: every lane of the wave
+    // passes the same `l`, which is then read from the first lane only.
+    template <bool UNIFORM>
+    GARG_DEV void run_loop(KerrLoop l, uint32_t budget) {
+        if constexpr (UNIFORM) l = wave_uniform(l);
+        const bool has_volume = l.gather != 0u;
+        const bool compensated = l.compensated != 0u;
         for (uint32_t n = 0; n < budget && !done; ++n) {
-            if (steps >= gs.step_cap) {
+            if (steps >= l.step_cap) {
                 done = true;
                 break;
             }
             float near_h = 0.2f * (r - k.rplus);
             float pole = fminf(fmaxf(e.st * (1.0f / GARG_F(KERR_POLE_SINE)), 0.05f), 1.0f);
-            float h = fminf(fmaxf(fminf(gs.step_factor * r, near_h) * pole, GARG_F(KERR_STEP_FLOOR)), gs.step_max);
+            float h = fminf(fmaxf(fminf(l.step_factor * r, near_h) * pole, GARG_F(KERR_STEP_FLOOR)), l.step_max);
             float hh = 0.5f * h;
             KerrEval k2 = kerr_eval(k, r + e.dr * hh, th + e.dth * hh, pr + e.dpr * hh, pth + e.dpth * hh);
             KerrEval k3 = kerr_eval(k, r + k2.dr * hh, th + k2.dth * hh, pr + k2.dpr * hh, pth + k2.dpth * hh);
@@ -709,17 +743,17 @@ struct KerrFastMarch {
                 float f = 1.0f - 2.0f * r * e.inv_sigma;
                 const KerrEval& ee = e;
                 float rr = r, pp = ph;
-                g.sample(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, fmaxf(f, 0.0f),
+                g.sample(l.volume, l.opacity, l.redshift != 0u, p, path, fmaxf(f, 0.0f),
                          [&]() { return kerr_direction_fast(rr, ee.st, ee.ct, pp, ee.dr, ee.dth, ee.dph); });
             }
             prev = p;
-            if (r >= gs.escape_radius) {
+            if (r >= l.escape_radius) {
                 outcome = GARG_RAY_ESCAPED;
                 done = true;
                 break;
             }
         }
-        if (!done && steps >= gs.step_cap) done = true;
+        if (!done && steps >= l.step_cap) done = true;
     }
 
     GARG_DEV bool falls_in() const {

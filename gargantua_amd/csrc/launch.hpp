@@ -42,6 +42,9 @@ struct BatchRequest {
     uint32_t tiles_x;     // 8 x 8 tiles per row of the image
     uint32_t ray_begin;   // first ray of this request, tile order
     uint64_t queue_begin; // offset of this request in the batch's ray queue
+    uint32_t ray_count;   // rays of this request in the batch; its queue
+                          // range is padded to a multiple of 64
+    uint32_t pad;
 };
 
 // Requests of one persistent launch; the LDS counter table bounds their number.
@@ -52,13 +55,15 @@ struct BatchArgs {
     uint32_t request_count;
     uint32_t requests_kerr;         // 1: every request has spin > 0; 0: every request has spin 0
     uint64_t total_rays;
-    unsigned long long* queue;      // eight {head, end} pairs, 128 bytes apart
+    unsigned long long* queue;      // refill policy: eight {head, end} pairs, 128
+                                    // bytes apart; tile policy: one head
     garg_march_counters* totals;
     uint32_t compaction_period;
     uint32_t refill_threshold;
 };
 
 hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
+uint32_t march_batch_resident_blocks(bool kerr, bool tiles, uint32_t block);
 
 // Particles, structure of arrays.
 struct ParticleView {

@@ -39,10 +39,12 @@ struct LaneRay {
     uint32_t pixel;  // py * width + px
 };
 
-// Tile-order ray of a request to its pixel; false for the padding rays of
-// edge tiles that fall outside the image.
+// Tile-order ray of a request to its pixel; false for the padding of the
+// request's queue range and for edge-tile rays outside the image.
 GARG_DEV bool ray_pixel(const BatchRequest& r, uint64_t g, uint32_t& px, uint32_t& py) {
-    uint32_t local = (uint32_t)(g - r.queue_begin) + r.ray_begin;
+    const uint32_t offset = (uint32_t)(g - r.queue_begin);
+    if (offset >= r.ray_count) return false;
+    uint32_t local = offset + r.ray_begin;
     uint32_t tile = local >> 6, lane = local & 63u;
     px = (tile % r.tiles_x) * 8u + (lane & 7u);
     py = (tile / r.tiles_x) * 8u + (lane >> 3);
@@ -79,8 +81,9 @@ struct Lane<true> {
         m.begin(u.spin, kerr_launch(u.spin, load3(u.camera_position), dir));
         return true;
     }
+    template <bool UNIFORM>
     GARG_DEV void run(const BatchRequest& r, uint32_t budget) {
-        m.run(r.uniforms, r.scene, true, r.uniforms.compensated != 0u, budget);
+        m.template run_loop<UNIFORM>(kerr_loop(r.uniforms, r.scene, true, r.uniforms.compensated != 0u), budget);
     }
     GARG_DEV bool done() const { return m.done; }
     GARG_DEV void finish(const BatchRequest& r, CounterTable& t, uint32_t request, uint32_t pixel) {
@@ -108,6 +111,7 @@ struct Lane<false> {
         m.begin(p.ray);
         return true;
     }
+    template <bool UNIFORM>
     GARG_DEV void run(const BatchRequest& r, uint32_t budget) {
         m.run(r.uniforms, r.scene, true, r.uniforms.compensated != 0u, budget);
     }
@@ -120,71 +124,17 @@ struct Lane<false> {
     }
 };
 
-template <bool KERR, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void march_batch_kernel(BatchArgs a) {
-    __shared__ uint32_t s_counts[GARG_BATCH_MAX_REQUESTS * 4];
-    __shared__ unsigned long long s_steps[GARG_BATCH_MAX_REQUESTS];
-    for (uint32_t i = threadIdx.x; i < a.request_count * 4u; i += BLOCK) s_counts[i] = 0u;
-    for (uint32_t i = threadIdx.x; i < a.request_count; i += BLOCK) s_steps[i] = 0ull;
+template <int BLOCK>
+GARG_DEV void clear_counters(uint32_t* s_counts, unsigned long long* s_steps, uint32_t requests) {
+    for (uint32_t i = threadIdx.x; i < requests * 4u; i += BLOCK) s_counts[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < requests; i += BLOCK) s_steps[i] = 0ull;
     __syncthreads();
-    CounterTable table{s_counts, s_steps};
+}
 
-    const uint32_t lane = __lane_id();
-    const uint64_t below = (1ull << lane) - 1ull;
-    const uint32_t home = blockIdx.x % XCDS;
-    uint32_t queue = home;
-    bool drained = false;
-
-    Lane<KERR> ray;
-    bool live = false;
-    LaneRay id{0u, 0u};
-
-    for (;;) {
-        uint64_t dead = __ballot(!live);
-        uint32_t live_count = 64u - (uint32_t)__popcll(dead);
-        if (!drained && dead != 0ull && live_count <= a.refill_threshold) {
-            // Claim up to one ray per dead lane, stealing once home is empty.
-            uint64_t want = (uint64_t)__popcll(dead);
-            uint64_t base = 0, got = 0;
-            while (!drained && got == 0) {
-                uint64_t b = 0;
-                if (lane == 0) b = atomicAdd(&a.queue[queue * QUEUE_STRIDE], (unsigned long long)want);
-                b = wave_broadcast(b);
-                uint64_t end = a.queue[queue * QUEUE_STRIDE + 1];
-                if (b < end) {
-                    base = b;
-                    got = min(want, end - b);
-                } else {
-                    queue = (queue + 1u) % XCDS;
-                    if (queue == home) drained = true;
-                }
-            }
-            uint64_t rank = (uint64_t)__popcll(dead & below);
-            if (!live && rank < got) {
-                uint64_t g = base + rank;
-                uint32_t r = find_request(a, g);
-                const BatchRequest& req = a.requests[r];
-                uint32_t px, py;
-                if (ray_pixel(req, g, px, py)) {
-                    id = LaneRay{r, py * req.uniforms.resolution[0] + px};
-                    live = ray.start(req, px, py, table, r, id.pixel);
-                }
-            }
-        }
-        if (__ballot(live) == 0ull) {
-            if (drained) break;
-            continue;
-        }
-        if (live) {
-            const BatchRequest& req = a.requests[id.request];
-            ray.run(req, a.compaction_period);
-            if (ray.done()) {
-                ray.finish(req, table, id.request, id.pixel);
-                live = false;
-            }
-        }
-    }
-
+// One global atomic per non-zero counter per block, into the request's
+// counters and the batch totals.
+template <int BLOCK>
+GARG_DEV void flush_counters(const BatchArgs& a, const uint32_t* s_counts, const unsigned long long* s_steps) {
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < a.request_count; i += BLOCK) {
         garg_march_counters* c = a.requests[i].counters;
@@ -204,6 +154,148 @@ __global__ __launch_bounds__(BLOCK) void march_batch_kernel(BatchArgs a) {
     }
 }
 
+// Claim up to `want` consecutive rays for the calling wave from its XCD's
+// range, moving on to the other ranges once that is empty. Returns the
+// count claimed (0 once every range is empty); wave-uniform.
+GARG_DEV uint64_t claim_rays(const BatchArgs& a, uint32_t home, uint32_t& queue, bool& drained, uint64_t want,
+                             uint64_t& base) {
+    const uint32_t lane = __lane_id();
+    while (!drained) {
+        uint64_t b = 0;
+        if (lane == 0) b = atomicAdd(&a.queue[queue * QUEUE_STRIDE], (unsigned long long)want);
+        b = wave_broadcast(b);
+        uint64_t end = a.queue[queue * QUEUE_STRIDE + 1];
+        if (b < end) {
+            base = b;
+            return min(want, end - b);
+        }
+        queue = (queue + 1u) % XCDS;
+        if (queue == home) drained = true;
+    }
+    return 0;
+}
+
+// Start the lane's ray g of the batch; false if it has nothing to march.
+template <bool KERR>
+GARG_DEV bool start_ray(const BatchArgs& a, Lane<KERR>& ray, LaneRay& id, CounterTable& table, uint64_t g) {
+    uint32_t r = find_request(a, g);
+    const BatchRequest& req = a.requests[r];
+    uint32_t px, py;
+    if (!ray_pixel(req, g, px, py)) return false;
+    id = LaneRay{r, py * req.uniforms.resolution[0] + px};
+    return ray.start(req, px, py, table, r, id.pixel);
+}
+
+// March the live lanes `budget` steps, one request at a time (a wave almost
+// always holds one): the request index is then wave-uniform, so its uniforms
+// and scene are scalar loads held in SGPRs, and one copy of the loop serves
+// every mix of requests. Finished rays are stored; returns whether the lane
+// is still live.
+template <bool KERR>
+GARG_DEV bool march_lanes(const BatchArgs& a, Lane<KERR>& ray, const LaneRay& id, CounterTable& table, bool live,
+                          uint32_t budget) {
+    bool pending = live;
+    for (;;) {
+        const uint64_t pending_mask = __ballot(pending);
+        if (pending_mask == 0ull) break;
+        const uint32_t lead = (uint32_t)__builtin_ctzll(pending_mask);
+        const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)id.request, (int)lead);
+        if (pending && id.request == r0) {
+            const BatchRequest& req = a.requests[r0];
+            ray.template run<true>(req, budget);
+            if (ray.done()) {
+                ray.finish(req, table, r0, id.pixel);
+                live = false;
+            }
+            pending = false;
+        }
+    }
+    return live;
+}
+
+// Lane-granular refill: a wave keeps marching while more than
+// `refill_threshold` lanes are live, then tops its finished lanes up.
+template <bool KERR, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void march_batch_kernel(BatchArgs a) {
+    __shared__ uint32_t s_counts[GARG_BATCH_MAX_REQUESTS * 4];
+    __shared__ unsigned long long s_steps[GARG_BATCH_MAX_REQUESTS];
+    clear_counters<BLOCK>(s_counts, s_steps, a.request_count);
+    CounterTable table{s_counts, s_steps};
+
+    const uint64_t below = (1ull << __lane_id()) - 1ull;
+    const uint32_t home = blockIdx.x % XCDS;
+    uint32_t queue = home;
+    bool drained = false;
+
+    Lane<KERR> ray;
+    bool live = false;
+    LaneRay id{0u, 0u};
+
+    for (;;) {
+        uint64_t dead = __ballot(!live);
+        uint32_t live_count = 64u - (uint32_t)__popcll(dead);
+        if (!drained && dead != 0ull && live_count <= a.refill_threshold) {
+            // One ray per dead lane, in lane order.
+            uint64_t base = 0;
+            uint64_t got = claim_rays(a, home, queue, drained, (uint64_t)__popcll(dead), base);
+            uint64_t rank = (uint64_t)__popcll(dead & below);
+            if (!live && rank < got) live = start_ray(a, ray, id, table, base + rank);
+        }
+        if (__ballot(live) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        live = march_lanes(a, ray, id, table, live, a.compaction_period);
+    }
+    flush_counters<BLOCK>(a, s_counts, s_steps);
+}
+
+// Tile-granular persistent march: a wave marches 64 consecutive rays (one
+// 8 x 8 tile) to the end, then takes the next 64. The first round is
+// static (wave w takes rays 64 w onward), so the launch does not open with
+// every wave queueing on one device-scope atomic; later rounds claim from a
+// single head in ray order, which finishes the frame in the order the
+// hardware would dispatch it and keeps the tail short. No ray state
+// outlives a tile, so the kernel needs no more registers than the
+// one-launch-per-frame march (five waves per SIMD): the policy for workloads
+// whose neighbouring rays take similar step counts.
+template <bool KERR, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void march_tiles_kernel(BatchArgs a) {
+    __shared__ uint32_t s_counts[GARG_BATCH_MAX_REQUESTS * 4];
+    __shared__ unsigned long long s_steps[GARG_BATCH_MAX_REQUESTS];
+    clear_counters<BLOCK>(s_counts, s_steps, a.request_count);
+    CounterTable table{s_counts, s_steps};
+
+    const uint32_t lane = __lane_id();
+    uint64_t base = ((uint64_t)blockIdx.x * (BLOCK / 64) + threadIdx.x / 64u) * 64u;
+    while (base < a.total_rays) {
+        const uint64_t got = min((uint64_t)64, a.total_rays - base);
+        // Request ranges are 64-aligned, so the whole tile is one request.
+        const uint32_t r = find_request(a, base);
+        const BatchRequest& req = a.requests[r];
+        Lane<KERR> ray;
+        uint32_t px = 0, py = 0;
+        bool live = lane < got && ray_pixel(req, base + lane, px, py);
+        const uint32_t pixel = py * req.uniforms.resolution[0] + px;
+        if (live) live = ray.start(req, px, py, table, r, pixel);
+        if (live) {
+            ray.template run<true>(req, 0xffffffffu);
+            ray.finish(req, table, r, pixel);
+        }
+        // Claiming only now, not before the march, keeps the claim's result
+        // out of the registers the march needs.
+        uint64_t next = 0;
+        if (lane == 0) next = atomicAdd(&a.queue[0], 64ull);
+        base = wave_broadcast(next);
+    }
+    flush_counters<BLOCK>(a, s_counts, s_steps);
+}
+
+// Tile policy: one head, past the rays of the static first round.
+__global__ void tile_queue_init_kernel(unsigned long long* queue, uint64_t first) {
+    if (threadIdx.x == 0) queue[0] = first;
+}
+
 __global__ void queue_init_kernel(unsigned long long* queue, uint64_t total) {
     uint32_t q = threadIdx.x;
     if (q >= XCDS) return;
@@ -217,17 +309,50 @@ __global__ void queue_init_kernel(unsigned long long* queue, uint64_t total) {
 
 template <bool KERR>
 static hipError_t launch_block(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream) {
+    const bool tiles = a.refill_threshold == 0u;
     switch (block) {
-        case 64: march_batch_kernel<KERR, 64><<<grid, 64, 0, stream>>>(a); break;
-        case 128: march_batch_kernel<KERR, 128><<<grid, 128, 0, stream>>>(a); break;
-        case 256: march_batch_kernel<KERR, 256><<<grid, 256, 0, stream>>>(a); break;
+        case 64:
+            if (tiles) march_tiles_kernel<KERR, 64><<<grid, 64, 0, stream>>>(a);
+            else march_batch_kernel<KERR, 64><<<grid, 64, 0, stream>>>(a);
+            break;
+        case 128:
+            if (tiles) march_tiles_kernel<KERR, 128><<<grid, 128, 0, stream>>>(a);
+            else march_batch_kernel<KERR, 128><<<grid, 128, 0, stream>>>(a);
+            break;
+        case 256:
+            if (tiles) march_tiles_kernel<KERR, 256><<<grid, 256, 0, stream>>>(a);
+            else march_batch_kernel<KERR, 256><<<grid, 256, 0, stream>>>(a);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+// Workgroups of the batched march one compute unit holds at once: a
+// persistent grid beyond that only queues workgroups that find the rays
+// already taken.
+uint32_t march_batch_resident_blocks(bool kerr, bool tiles, uint32_t block) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define GARG_OCC(K, B)                                                                                    \
+    e = tiles ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_tiles_kernel<K, B>, B, 0)         \
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_batch_kernel<K, B>, B, 0)
+    if (kerr) {
+        if (block == 64) GARG_OCC(true, 64);
+        else if (block == 128) GARG_OCC(true, 128);
+        else if (block == 256) GARG_OCC(true, 256);
+    } else {
+        if (block == 64) GARG_OCC(false, 64);
+        else if (block == 128) GARG_OCC(false, 128);
+        else if (block == 256) GARG_OCC(false, 256);
+    }
+#undef GARG_OCC
+    return e == hipSuccess && n > 0 ? (uint32_t)n : 0u;
+}
+
 hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream) {
-    queue_init_kernel<<<1, 64, 0, stream>>>(a.queue, a.total_rays);
+    if (a.refill_threshold == 0u) tile_queue_init_kernel<<<1, 64, 0, stream>>>(a.queue, (uint64_t)grid * block);
+    else queue_init_kernel<<<1, 64, 0, stream>>>(a.queue, a.total_rays);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return a.requests_kerr ? launch_block<true>(a, grid, block, stream) : launch_block<false>(a, grid, block, stream);

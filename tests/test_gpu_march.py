@@ -57,11 +57,13 @@ def same_bits(a, b):
                               if y.dtype == np.float32 else y) for x, y in zip(a, b))
 
 
-def test_batched_matches_plain_bitwise_kerr(ctx):
+@pytest.mark.parametrize("threshold", [40, 0])
+def test_batched_matches_plain_bitwise_kerr(ctx, threshold):
+    """Lane refill (threshold > 0) and whole-tile claims (threshold 0)."""
     vol = disk_volume(ctx)
     req = kerr_request(ctx, 256, 144, 0.9, 60.0, 50.0, 4096, vol)
     (plain, pc) = render_plain(ctx, req)
-    total = ctx.march_batch([req], ctx.batch_config(compaction_period=8, refill_threshold=40))
+    total = ctx.march_batch([req], ctx.batch_config(compaction_period=8, refill_threshold=threshold))
     batched = req["targets"].read()
     assert same_bits(plain, batched)
     bc = req["targets"].counters()
@@ -71,12 +73,13 @@ def test_batched_matches_plain_bitwise_kerr(ctx):
     print(f"rays {pc.rays} steps {pc.steps} captured {pc.captured}")
 
 
-def test_batched_matches_plain_bitwise_schwarzschild_with_jump(ctx):
+@pytest.mark.parametrize("threshold", [32, 0])
+def test_batched_matches_plain_bitwise_schwarzschild_with_jump(ctx, threshold):
     vol = disk_volume(ctx)
     tables = api.Tables(ctx)
     req = schw_request(ctx, 192, 108, vol, tables)
     (plain, pc) = render_plain(ctx, req)
-    ctx.march_batch([req], ctx.batch_config(compaction_period=16, refill_threshold=32))
+    ctx.march_batch([req], ctx.batch_config(compaction_period=16, refill_threshold=threshold))
     assert same_bits(plain, req["targets"].read())
     assert req["targets"].counters().steps == pc.steps
 
