@@ -31,6 +31,10 @@ METRIC = "RK4 geodesic steps/s at 1080p a=0.9 vs plain march; p50/p99 render lat
 FLOPS_PER_KERR_STEP = 509
 FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak (packed)
 FP32_PEAK_NONPACKED_TFLOPS = 78.6
+# HBM bytes per launch of the batched Kerr march at this frame, from two
+# rocprofv3 counter runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a
+# 256 MiB device copy reads); profiles/march_kerr_1080p_a0.9_counters.md.
+TRAFFIC_BYTES_PER_LAUNCH = 481.5e6
 
 
 def parse():
@@ -197,7 +201,8 @@ def main():
         "latency_ms": {"p50": float(np.percentile(batched_frames, 50)),
                        "p99": float(np.percentile(batched_frames, 99))},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+                     "traffic_unit": "bytes per launch",
                      "peak_nonpacked": FP32_PEAK_NONPACKED_TFLOPS,
                      "flops_per_step": FLOPS_PER_KERR_STEP,
                      "kernel": ("march_tiles_kernel" if cfg.refill_threshold == 0 else "march_batch_kernel")
