@@ -8,7 +8,7 @@
  *
  * It is a from-scratch restatement of the reference's Swift oracle
  * (the Swift files under Sources/Oracle) plus the kernel-only semantics the Swift oracle
- * lacks (fixed point deposit, resolve, sphere jump, gather, starfield, bake),
+ * lacks (fixed point deposit, resolve, sphere jump, gather, starfield),
  * each function citing the reference file:line it follows. The reference's
  * own oracle cannot be built here (Swift is absent), so this restatement is
  * pinned by the reference's known-answer tests and goldens and by closed
