@@ -57,9 +57,12 @@ struct garg_ctx {
     float* inverse_cdf = nullptr;
     double temperature_scale = 0.0;
     // Scratch of the batched march and the probes.
+    // Two halves each (spinning pass, Schwarzschild pass), so both passes of
+    // one call enqueue without waiting for each other.
     BatchRequest* batch_requests = nullptr;
     BatchRequest* batch_staging = nullptr;  // pinned
-    hipEvent_t batch_staging_free = nullptr;
+    hipEvent_t batch_staging_free[2] = {nullptr, nullptr};
+    hipEvent_t batch_done = nullptr;        // the previous call's launches have run
     unsigned long long* queue = nullptr;
     garg_march_counters* batch_totals = nullptr;
     garg_march_counters* scratch_counters = nullptr;
@@ -232,13 +235,15 @@ int garg_ctx_create(int device, uint64_t budget_bytes, garg_ctx** out) {
     if ((rc = ctx_alloc(ctx, cdf.size() * 4, (void**)&ctx->inverse_cdf, "the seeding table"))) return rc;
     HIP_TRY(hipMemcpy(ctx->blackbody, bb.data(), bb.size() * 4, hipMemcpyHostToDevice), "uploading the blackbody table");
     HIP_TRY(hipMemcpy(ctx->inverse_cdf, cdf.data(), cdf.size() * 4, hipMemcpyHostToDevice), "uploading the seeding table");
-    if ((rc = ctx_alloc(ctx, sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS, (void**)&ctx->batch_requests,
+    if ((rc = ctx_alloc(ctx, 2 * sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS, (void**)&ctx->batch_requests,
                         "the batch request table")))
         return rc;
-    HIP_TRY(hipHostMalloc((void**)&ctx->batch_staging, sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS, 0),
+    HIP_TRY(hipHostMalloc((void**)&ctx->batch_staging, 2 * sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS, 0),
             "allocating pinned staging");
-    HIP_TRY(hipEventCreateWithFlags(&ctx->batch_staging_free, hipEventDisableTiming), "creating an event");
-    if ((rc = ctx_alloc(ctx, 8 * 16 * sizeof(unsigned long long), (void**)&ctx->queue, "the ray queue"))) return rc;
+    for (hipEvent_t& e : ctx->batch_staging_free)
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "creating an event");
+    HIP_TRY(hipEventCreateWithFlags(&ctx->batch_done, hipEventDisableTiming), "creating an event");
+    if ((rc = ctx_alloc(ctx, 2 * 8 * 16 * sizeof(unsigned long long), (void**)&ctx->queue, "the ray queue"))) return rc;
     if ((rc = ctx_alloc(ctx, sizeof(garg_march_counters), (void**)&ctx->batch_totals, "batch counters"))) return rc;
     if ((rc = ctx_alloc(ctx, sizeof(garg_march_counters), (void**)&ctx->scratch_counters, "counters"))) return rc;
     *out = ctx;
@@ -250,13 +255,15 @@ int garg_ctx_destroy(garg_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     ctx_free(ctx, ctx->blackbody, BLACKBODY_TABLE_SIZE * 16);
     ctx_free(ctx, ctx->inverse_cdf, INVERSE_CDF_COUNT * 4);
-    ctx_free(ctx, ctx->batch_requests, sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS);
-    ctx_free(ctx, ctx->queue, 8 * 16 * sizeof(unsigned long long));
+    ctx_free(ctx, ctx->batch_requests, 2 * sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS);
+    ctx_free(ctx, ctx->queue, 2 * 8 * 16 * sizeof(unsigned long long));
     ctx_free(ctx, ctx->batch_totals, sizeof(garg_march_counters));
     ctx_free(ctx, ctx->scratch_counters, sizeof(garg_march_counters));
     ctx_free(ctx, ctx->probe_scratch, ctx->probe_scratch_bytes);
     if (ctx->batch_staging) (void)hipHostFree(ctx->batch_staging);
-    if (ctx->batch_staging_free) (void)hipEventDestroy(ctx->batch_staging_free);
+    for (hipEvent_t e : ctx->batch_staging_free)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->batch_done) (void)hipEventDestroy(ctx->batch_done);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return GARG_OK;
@@ -806,24 +813,30 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
                 "request %u starts past its last ray.", i);
     }
     hipStream_t s = stream_of(ctx, stream);
+    // The device tables, queues and totals are reused: a call on another
+    // stream first waits for the previous call's launches.
+    HIP_TRY(hipStreamWaitEvent(s, ctx->batch_done, 0), "ordering after the previous batch");
     HIP_TRY(launch_fill_zero(ctx->batch_totals, sizeof(garg_march_counters), s), "clearing batch counters");
     uint32_t waves = config->waves_per_simd ? config->waves_per_simd : 4u;
     uint32_t grid = std::max(1u, ctx->info.compute_units * 4u * waves / (config->block / 64u));
-    for (int kerr = 1; kerr >= 0; --kerr) {
-        // The staging buffer is reused only once the previous copy from it ran.
-        HIP_TRY(hipEventSynchronize(ctx->batch_staging_free), "waiting for the staging buffer");
+    for (int part = 0; part < 2; ++part) {
+        const bool kerr = part == 0;
+        // This half of the pinned staging is reused once its previous copy ran.
+        HIP_TRY(hipEventSynchronize(ctx->batch_staging_free[part]), "waiting for the staging buffer");
+        BatchRequest* staging = ctx->batch_staging + (size_t)part * GARG_BATCH_MAX_REQUESTS;
+        BatchRequest* table = ctx->batch_requests + (size_t)part * GARG_BATCH_MAX_REQUESTS;
         uint32_t n = 0;
         uint64_t queue = 0;
         for (uint32_t i = 0; i < count; ++i) {
             const garg_march_request& r = requests[i];
-            if ((r.uniforms.spin > 0.0f) != (kerr == 1)) continue;
+            if ((r.uniforms.spin > 0.0f) != kerr) continue;
             uint32_t all = garg_request_rays(r.uniforms.resolution[0], r.uniforms.resolution[1]);
             uint32_t rays = all - r.ray_begin;
             if (r.ray_count) rays = std::min(rays, r.ray_count);
             if (rays == 0) continue;
-            BatchRequest& b = ctx->batch_staging[n++];
+            BatchRequest& b = staging[n++];
             b.uniforms = device_uniforms(&r.uniforms, r.volume);
-            b.scene = make_scene(ctx, r.volume, r.uniforms.spin > 0.0f ? nullptr : r.tables);
+            b.scene = make_scene(ctx, r.volume, kerr ? nullptr : r.tables);
             b.color = r.targets->color;
             b.debug = r.targets->debug;
             b.depth = r.targets->depth;
@@ -837,26 +850,22 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
             queue += ((uint64_t)rays + 63u) & ~(uint64_t)63u;
         }
         if (n == 0) continue;
-        HIP_TRY(hipMemcpyAsync(ctx->batch_requests, ctx->batch_staging, sizeof(BatchRequest) * n, hipMemcpyHostToDevice, s),
-                "uploading the batch");
-        HIP_TRY(hipEventRecord(ctx->batch_staging_free, s), "recording an event");
+        HIP_TRY(hipMemcpyAsync(table, staging, sizeof(BatchRequest) * n, hipMemcpyHostToDevice, s), "uploading the batch");
+        HIP_TRY(hipEventRecord(ctx->batch_staging_free[part], s), "recording an event");
         BatchArgs a;
-        a.requests = ctx->batch_requests;
+        a.requests = table;
         a.request_count = n;
-        a.requests_kerr = (uint32_t)kerr;
+        a.requests_kerr = kerr ? 1u : 0u;
         a.total_rays = queue;
-        a.queue = ctx->queue;
+        a.queue = ctx->queue + (size_t)part * 8 * 16;
         a.totals = ctx->batch_totals;
         a.compaction_period = config->compaction_period;
         a.refill_threshold = config->refill_threshold;
-        uint32_t resident = march_batch_resident_blocks(kerr == 1, config->refill_threshold == 0u, config->block);
+        uint32_t resident = march_batch_resident_blocks(kerr, config->refill_threshold == 0u, config->block);
         uint32_t launch_grid = resident ? std::min(grid, resident * ctx->info.compute_units) : grid;
         HIP_TRY(launch_march_batch(a, launch_grid, config->block, s), "launching the batched march");
-        if (kerr == 1) {
-            // Both passes share the request table; the second upload waits for the first launch.
-            HIP_TRY(hipStreamSynchronize(s), "waiting for the spinning pass");
-        }
     }
+    HIP_TRY(hipEventRecord(ctx->batch_done, s), "recording an event");
     if (total) {
         HIP_TRY(hipMemcpyAsync(total, ctx->batch_totals, sizeof(garg_march_counters), hipMemcpyDeviceToHost, s),
                 "reading batch counters");

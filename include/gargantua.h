@@ -255,17 +255,21 @@ typedef struct {
 } garg_march_request;
 
 typedef struct {
-    uint32_t block;             /* threads per workgroup, multiple of 64 */
-    uint32_t waves_per_simd;    /* resident waves per SIMD for the persistent grid; 0 = 4 */
+    uint32_t block;             /* threads per workgroup: 64, 128 or 256 */
+    uint32_t waves_per_simd;    /* waves per SIMD of the persistent grid (capped at what
+                                   fits at once); 0 = 4 */
     uint32_t compaction_period; /* RK4 steps between refill checks */
-    uint32_t refill_threshold;  /* refill a wave when at most this many of 64 lanes are live */
+    uint32_t refill_threshold;  /* refill a wave when at most this many of 64 lanes are
+                                   live; 0 = whole-tile claims: a wave marches 64 rays to
+                                   the end before it takes the next 64 */
 } garg_batch_config;
 
 /* Rays of a request in tile order: 8x8 pixel tiles row-major, 64 rays each. */
 uint32_t garg_request_rays(uint32_t width, uint32_t height);
 int garg_batch_config_default(garg_ctx* ctx, garg_batch_config* out);
 /* Persistent launches marching every requested ray of every request with
- * work stealing and live-ray refill: one launch for the spinning requests
+ * work stealing and either live-ray refill or whole-tile claims (see
+ * garg_batch_config): one launch for the spinning requests
  * and one for the Schwarzschild ones, at most 256 requests. Each request's
  * targets counters accumulate; `total` (optional) receives the sums over
  * the call, read back synchronously. */
