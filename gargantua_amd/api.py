@@ -28,6 +28,8 @@ RAY_CAPTURED, RAY_ESCAPED, RAY_EXHAUSTED = 0, 1, 2
 POTENTIAL_NEWTONIAN, POTENTIAL_PACZYNSKI_WIITA = 0, 1
 GENERATOR_DISK, GENERATOR_ANNULUS = 0, 1
 INTEGRATOR_REFERENCE, INTEGRATOR_FAST = 0, 1
+GARG_E_ARGUMENT, GARG_E_DEVICE, GARG_E_BUDGET, GARG_E_STATE = 1, 2, 3, 4
+FRAME_AUTO = 0xFFFFFFFF  # splat frame from the particle store's dispatch count
 
 
 class GargError(RuntimeError):
@@ -141,6 +143,11 @@ SIGNATURES = {
     "garg_tonemap": (C.c_int, [vp, vp, f32, u32, f32, vp]),
     "garg_deflection_table": (C.c_int, [vp, vp, f64, u32, f64, f64, u32, vp, vp, vp]),
     "garg_shadow_boundary": (C.c_int, [vp, vp, f64, f64, f64, u32, u32, u32, u32, vp, vp]),
+    "garg_particles_dispatches": (C.c_int, [vp, P(u32)]),
+    "garg_graph_begin": (C.c_int, [vp, vp]),
+    "garg_graph_end": (C.c_int, [vp, vp, P(vp)]),
+    "garg_graph_launch": (C.c_int, [vp, vp]),
+    "garg_graph_destroy": (C.c_int, [vp]),
 }
 
 _lib = None
@@ -267,6 +274,18 @@ class Context:
     def event(self) -> "Event":
         return Event(self)
 
+    def stream(self) -> "Stream":
+        return Stream(self)
+
+    def graph_begin(self, stream=None):
+        """Record every launch enqueued on `stream` until graph_end."""
+        _check(self.lib.garg_graph_begin(self.h, stream))
+
+    def graph_end(self, stream=None) -> "Graph":
+        h = vp()
+        _check(self.lib.garg_graph_end(self.h, stream, C.byref(h)))
+        return Graph(self, h)
+
     def probe(self, probes, u: MarchUniforms, tables=None, integrator: int = INTEGRATOR_FAST):
         n = len(probes)
         arr = (RayProbe * n)()
@@ -345,6 +364,40 @@ class Event:
             self.h = None
 
 
+class Stream:
+    """A HIP stream of the context; pass it wherever an entry point takes `stream`."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = vp()
+        _check(ctx.lib.garg_stream_create(ctx.h, C.byref(h)))
+        self.h = h
+        self._as_param_ = h
+
+    def sync(self):
+        _check(self.ctx.lib.garg_stream_sync(self.ctx.h, self.h))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.garg_stream_destroy(self.h)
+            self.h = None
+
+
+class Graph:
+    """An instantiated capture (garg_graph_end); launch replays it."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def launch(self, stream=None):
+        _check(self.ctx.lib.garg_graph_launch(self.h, stream))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.garg_graph_destroy(self.h)
+            self.h = None
+
+
 class Particles:
     def __init__(self, ctx: Context, count: int, seed: int = 1, potential: int = POTENTIAL_PACZYNSKI_WIITA):
         self.ctx, self.count = ctx, count
@@ -371,6 +424,11 @@ class Particles:
     def checksum(self) -> int:
         v = u64()
         _check(self.ctx.lib.garg_particles_checksum(self.h, C.byref(v)))
+        return v.value
+
+    def dispatches(self) -> int:
+        v = u32()
+        _check(self.ctx.lib.garg_particles_dispatches(self.h, C.byref(v)))
         return v.value
 
     def close(self):

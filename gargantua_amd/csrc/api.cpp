@@ -158,6 +158,21 @@ int sync_stream(hipStream_t s, const char* what) {
     return GARG_OK;
 }
 
+bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus status = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &status) == hipSuccess && status == hipStreamCaptureStatusActive;
+}
+
+// Entry points that copy to or from the host, or wait for the device,
+// cannot be recorded into a graph: they refuse while either the stream they
+// were given or the context's stream is capturing.
+#define REFUSE_WHILE_CAPTURING(CTX_, STREAM_, what)                                                      \
+    do {                                                                                                 \
+        if (capturing((CTX_)->stream) || ((STREAM_) && capturing((STREAM_)->s)))                          \
+            return fail(GARG_E_STATE, "%s copies to or from the host or waits for the device, so it cannot " \
+                        "run while a graph is being captured.", what);                                    \
+    } while (0)
+
 Scene make_scene(garg_ctx* ctx, garg_volume* volume, garg_tables* tables) {
     Scene sc{};
     if (volume)
@@ -252,6 +267,7 @@ int garg_ctx_create(int device, uint64_t budget_bytes, garg_ctx** out) {
 
 int garg_ctx_destroy(garg_ctx* ctx) {
     if (!ctx) return GARG_OK;
+    REFUSE_WHILE_CAPTURING(ctx, (garg_stream*)nullptr, "garg_ctx_destroy");
     (void)hipStreamSynchronize(ctx->stream);
     ctx_free(ctx, ctx->blackbody, BLACKBODY_TABLE_SIZE * 16);
     ctx_free(ctx, ctx->inverse_cdf, INVERSE_CDF_COUNT * 4);
@@ -303,6 +319,7 @@ int garg_stream_destroy(garg_stream* stream) {
 
 int garg_stream_sync(garg_ctx* ctx, garg_stream* stream) {
     REQUIRE(ctx, "ctx is required.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_stream_sync");
     return sync_stream(stream_of(ctx, stream), "waiting for the stream");
 }
 
@@ -351,7 +368,7 @@ int garg_particles_create(garg_ctx* ctx, uint32_t count, uint64_t seed, uint32_t
     p->seed = seed;
     p->potential = potential;
     uint64_t field = align256((uint64_t)count * 4u);
-    p->bytes = field * 8u;
+    p->bytes = field * 8u + 256u;
     int rc = ctx_alloc(ctx, p->bytes, &p->base, "the particle store");
     if (rc) {
         delete p;
@@ -360,13 +377,22 @@ int garg_particles_create(garg_ctx* ctx, uint32_t count, uint64_t seed, uint32_t
     char* b = static_cast<char*>(p->base);
     p->view = ParticleView{(float*)(b), (float*)(b + field), (float*)(b + 2 * field), (float*)(b + 3 * field),
                            (float*)(b + 4 * field), (float*)(b + 5 * field), (float*)(b + 6 * field),
-                           (uint32_t*)(b + 7 * field), count};
+                           (uint32_t*)(b + 7 * field), (uint32_t*)(b + 8 * field), count};
+    // The dispatch count starts at zero even before the first seed.
+    rc = launch_fill_zero(p->view.dispatches, 256, ctx->stream) == hipSuccess ? GARG_OK
+                                                                               : fail(GARG_E_DEVICE, "clearing the dispatch count");
+    if (rc) {
+        ctx_free(ctx, p->base, p->bytes);
+        delete p;
+        return rc;
+    }
     *out = p;
     return GARG_OK;
 }
 
 int garg_particles_destroy(garg_particles* p) {
     if (!p) return GARG_OK;
+    REFUSE_WHILE_CAPTURING(p->ctx, (garg_stream*)nullptr, "garg_particles_destroy");
     (void)hipStreamSynchronize(p->ctx->stream);
     ctx_free(p->ctx, p->base, p->bytes);
     delete p;
@@ -414,6 +440,7 @@ int garg_particles_reseed(garg_particles* p, uint64_t seed) {
 
 int garg_particles_read(garg_particles* p, garg_particle* host, uint32_t count) {
     REQUIRE(p && host, "particles and host are required.");
+    REFUSE_WHILE_CAPTURING(p->ctx, (garg_stream*)nullptr, "garg_particles_read");
     REQUIRE(count == p->count, "the store holds %u particles, not %u.", p->count, count);
     garg_particle* tmp = nullptr;
     int rc = ctx_alloc(p->ctx, (uint64_t)count * sizeof(garg_particle), (void**)&tmp, "the particle readback");
@@ -429,6 +456,7 @@ int garg_particles_read(garg_particles* p, garg_particle* host, uint32_t count) 
 
 int garg_particles_write(garg_particles* p, const garg_particle* host, uint32_t count) {
     REQUIRE(p && host, "particles and host are required.");
+    REFUSE_WHILE_CAPTURING(p->ctx, (garg_stream*)nullptr, "garg_particles_write");
     REQUIRE(count == p->count, "the store holds %u particles, not %u.", p->count, count);
     garg_particle* tmp = nullptr;
     int rc = ctx_alloc(p->ctx, (uint64_t)count * sizeof(garg_particle), (void**)&tmp, "the particle upload");
@@ -444,6 +472,7 @@ int garg_particles_write(garg_particles* p, const garg_particle* host, uint32_t 
 
 int garg_particles_checksum(garg_particles* p, uint64_t* fnv1a) {
     REQUIRE(p && fnv1a, "particles and fnv1a are required.");
+    REFUSE_WHILE_CAPTURING(p->ctx, (garg_stream*)nullptr, "garg_particles_checksum");
     std::vector<garg_particle> host(p->count);
     int rc = garg_particles_read(p, host.data(), p->count);
     if (rc) return rc;
@@ -490,6 +519,7 @@ int garg_volume_create(garg_ctx* ctx, uint32_t size, float half_extent, float ha
 
 int garg_volume_destroy(garg_volume* v) {
     if (!v) return GARG_OK;
+    REFUSE_WHILE_CAPTURING(v->ctx, (garg_stream*)nullptr, "garg_volume_destroy");
     (void)hipStreamSynchronize(v->ctx->stream);
     ctx_free(v->ctx, v->base, v->bytes);
     delete v;
@@ -520,6 +550,7 @@ int garg_splat_bin(garg_volume* v, garg_particles* p, garg_stream* stream, uint3
     a.frame = frame;
     a.seed_lo = (uint32_t)p->seed;
     a.seed_hi = (uint32_t)(p->seed >> 32);
+    a.frame_counter = frame == GARG_FRAME_AUTO ? p->view.dispatches : nullptr;
     HIP_TRY(launch_splat_bin(a, stream_of(v->ctx, stream)), "launching splatParticles");
     return GARG_OK;
 }
@@ -543,6 +574,7 @@ int garg_splat_resolve(garg_volume* v, garg_stream* stream, uint32_t clear_sums)
 
 int garg_volume_read_sums(garg_volume* v, uint32_t* emission, int32_t* momentum, uint32_t* mass) {
     REQUIRE(v, "volume is required.");
+    REFUSE_WHILE_CAPTURING(v->ctx, (garg_stream*)nullptr, "garg_volume_read_sums");
     uint64_t voxels = (uint64_t)v->size * v->size * v->size;
     hipStream_t s = v->ctx->stream;
     if (emission) HIP_TRY(hipMemcpyAsync(emission, v->emission, voxels * 8u, hipMemcpyDeviceToHost, s), "reading sums");
@@ -553,6 +585,7 @@ int garg_volume_read_sums(garg_volume* v, uint32_t* emission, int32_t* momentum,
 
 int garg_volume_read_fields(garg_volume* v, uint16_t* emission, uint16_t* velocity) {
     REQUIRE(v, "volume is required.");
+    REFUSE_WHILE_CAPTURING(v->ctx, (garg_stream*)nullptr, "garg_volume_read_fields");
     uint64_t voxels = (uint64_t)v->size * v->size * v->size;
     hipStream_t s = v->ctx->stream;
     if (emission) HIP_TRY(hipMemcpy2DAsync(emission, 8, v->voxels, 16, 8, voxels, hipMemcpyDeviceToHost, s), "reading fields");
@@ -564,6 +597,7 @@ int garg_volume_read_fields(garg_volume* v, uint16_t* emission, uint16_t* veloci
 
 int garg_volume_write_fields(garg_volume* v, const uint16_t* emission, const uint16_t* velocity) {
     REQUIRE(v && emission && velocity, "volume, emission and velocity are required.");
+    REFUSE_WHILE_CAPTURING(v->ctx, (garg_stream*)nullptr, "garg_volume_write_fields");
     uint64_t voxels = (uint64_t)v->size * v->size * v->size;
     hipStream_t s = v->ctx->stream;
     HIP_TRY(hipMemcpy2DAsync(v->voxels, 16, emission, 8, 8, voxels, hipMemcpyHostToDevice, s), "writing fields");
@@ -576,6 +610,7 @@ int garg_volume_write_fields(garg_volume* v, const uint16_t* emission, const uin
 
 int garg_tables_create(garg_ctx* ctx, garg_stream* stream, float sphere_radius, garg_tables** out) {
     REQUIRE(ctx && out, "ctx and out are required.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_tables_create");
     REQUIRE(sphere_radius > 4.0f, "the integration sphere must lie outside r = 4.");
     garg_tables* t = new garg_tables();
     t->ctx = ctx;
@@ -601,6 +636,7 @@ int garg_tables_create(garg_ctx* ctx, garg_stream* stream, float sphere_radius, 
 
 int garg_tables_destroy(garg_tables* t) {
     if (!t) return GARG_OK;
+    REFUSE_WHILE_CAPTURING(t->ctx, (garg_stream*)nullptr, "garg_tables_destroy");
     (void)hipStreamSynchronize(t->ctx->stream);
     ctx_free(t->ctx, t->data, 3u * SWEEP_TABLE_SIZE * 4u);
     delete t;
@@ -609,6 +645,7 @@ int garg_tables_destroy(garg_tables* t) {
 
 int garg_tables_update(garg_tables* t, garg_stream* stream, float camera_radius) {
     REQUIRE(t, "tables are required.");
+    REFUSE_WHILE_CAPTURING(t->ctx, stream, "garg_tables_update");
     REQUIRE(camera_radius > 3.5f, "the camera radius must exceed 3.5.");
     double r = (double)camera_radius;
     if (std::fabs(r - t->camera_radius) <= 1e-6) return GARG_OK;
@@ -640,6 +677,7 @@ int garg_tables_apply(garg_tables* t, garg_march_uniforms* u) {
 
 int garg_tables_read(garg_tables* t, float* sphere, float* camera, float* sky) {
     REQUIRE(t, "tables are required.");
+    REFUSE_WHILE_CAPTURING(t->ctx, (garg_stream*)nullptr, "garg_tables_read");
     hipStream_t s = t->ctx->stream;
     HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
     if (sphere) HIP_TRY(hipMemcpyAsync(sphere, t->data, SWEEP_TABLE_SIZE * 4u, hipMemcpyDeviceToHost, s), "reading tables");
@@ -684,6 +722,7 @@ int garg_targets_create(garg_ctx* ctx, uint32_t width, uint32_t height, garg_tar
 
 int garg_targets_destroy(garg_targets* t) {
     if (!t) return GARG_OK;
+    REFUSE_WHILE_CAPTURING(t->ctx, (garg_stream*)nullptr, "garg_targets_destroy");
     (void)hipStreamSynchronize(t->ctx->stream);
     ctx_free(t->ctx, t->base, t->bytes);
     delete t;
@@ -692,6 +731,7 @@ int garg_targets_destroy(garg_targets* t) {
 
 int garg_targets_read(garg_targets* t, uint16_t* color, uint16_t* debug, float* depth) {
     REQUIRE(t, "targets are required.");
+    REFUSE_WHILE_CAPTURING(t->ctx, (garg_stream*)nullptr, "garg_targets_read");
     HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
     uint64_t px = (uint64_t)t->width * t->height;
     if (color) HIP_TRY(hipMemcpy(color, t->color, px * 8u, hipMemcpyDeviceToHost), "reading the colour target");
@@ -702,6 +742,7 @@ int garg_targets_read(garg_targets* t, uint16_t* color, uint16_t* debug, float* 
 
 int garg_targets_read_counters(garg_targets* t, garg_march_counters* out) {
     REQUIRE(t && out, "targets and out are required.");
+    REFUSE_WHILE_CAPTURING(t->ctx, (garg_stream*)nullptr, "garg_targets_read_counters");
     HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
     HIP_TRY(hipMemcpy(out, t->counters, sizeof(garg_march_counters), hipMemcpyDeviceToHost), "reading counters");
     return GARG_OK;
@@ -748,6 +789,7 @@ int garg_probe(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes,
                const garg_march_uniforms* uniforms, garg_tables* tables, uint32_t integrator,
                garg_ray_probe_result* results) {
     REQUIRE(ctx && probes && results, "ctx, probes and results are required.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_probe");
     REQUIRE(count > 0, "no probes.");
     REQUIRE(integrator <= 1u, "unknown integrator %u.", integrator);
     REQUIRE(uniforms && uniforms->spin >= 0.0f && uniforms->spin < 1.0f, "spin must be in [0, 1).");
@@ -795,6 +837,7 @@ int garg_batch_config_default(garg_ctx* ctx, garg_batch_config* out) {
 int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_request* requests, uint32_t count,
                      const garg_batch_config* config, garg_march_counters* total) {
     REQUIRE(ctx && requests && config, "ctx, requests and config are required.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_march_batch");
     REQUIRE(count > 0 && count <= GARG_BATCH_MAX_REQUESTS, "a batch holds 1 to %d requests, not %u.",
             GARG_BATCH_MAX_REQUESTS, count);
     REQUIRE(config->block == 64 || config->block == 128 || config->block == 256, "block must be 64, 128 or 256.");
@@ -909,6 +952,7 @@ int garg_bake_create(garg_ctx* ctx, uint32_t width, uint32_t height, uint32_t ca
 
 int garg_bake_destroy(garg_bake* b) {
     if (!b) return GARG_OK;
+    REFUSE_WHILE_CAPTURING(b->ctx, (garg_stream*)nullptr, "garg_bake_destroy");
     (void)hipStreamSynchronize(b->ctx->stream);
     ctx_free(b->ctx, b->base, b->bytes);
     delete b;
@@ -961,6 +1005,7 @@ int garg_bake_walk(garg_bake* b, garg_stream* stream, const garg_march_uniforms*
 
 int garg_bake_overflow(garg_bake* b, uint32_t* rays) {
     REQUIRE(b && rays, "bake and rays are required.");
+    REFUSE_WHILE_CAPTURING(b->ctx, (garg_stream*)nullptr, "garg_bake_overflow");
     HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
     HIP_TRY(hipMemcpy(rays, b->overflow, 4, hipMemcpyDeviceToHost), "reading the overflow count");
     return GARG_OK;
@@ -968,6 +1013,7 @@ int garg_bake_overflow(garg_bake* b, uint32_t* rays) {
 
 int garg_bake_read(garg_bake* b, uint32_t* headers, uint16_t* samples) {
     REQUIRE(b, "bake is required.");
+    REFUSE_WHILE_CAPTURING(b->ctx, (garg_stream*)nullptr, "garg_bake_read");
     HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
     uint64_t rays = (uint64_t)b->width * b->height;
     if (headers) HIP_TRY(hipMemcpy(headers, b->headers, rays * 16u, hipMemcpyDeviceToHost), "reading bake headers");
@@ -981,6 +1027,7 @@ int garg_bake_read(garg_bake* b, uint32_t* headers, uint16_t* samples) {
 int garg_tonemap(garg_targets* t, garg_stream* stream, float exposure, uint32_t debug_view, float drift_budget,
                  uint8_t* rgba8) {
     REQUIRE(t && rgba8, "targets and rgba8 are required.");
+    REFUSE_WHILE_CAPTURING(t->ctx, stream, "garg_tonemap");
     REQUIRE(drift_budget > 0.0f || !debug_view, "the debug view needs a positive drift budget.");
     hipStream_t s = stream_of(t->ctx, stream);
     TonemapArgs a{t->color, t->debug, t->rgba8, t->width * t->height, exposure, debug_view ? 1u : 0u, drift_budget};
@@ -995,6 +1042,7 @@ int garg_tonemap(garg_targets* t, garg_stream* stream, float exposure, uint32_t 
 int garg_deflection_table(garg_ctx* ctx, garg_stream* stream, double r_obs, uint32_t count, double b_lo, double b_hi,
                           uint32_t intervals, double* b_out, double* alpha_out, double* sweep_out) {
     REQUIRE(ctx && b_out && alpha_out && sweep_out, "ctx and the three outputs are required.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_deflection_table");
     REQUIRE(count >= 2 && count <= (1u << 20), "count must be in [2, 2^20].");
     REQUIRE(b_lo > B_CRITICAL && b_hi > b_lo, "the impact parameters must lie above 3 sqrt(3) and increase.");
     REQUIRE(r_obs > 3.0 && intervals > 0, "r_obs must exceed 3 and intervals must be positive.");
@@ -1016,6 +1064,7 @@ int garg_shadow_boundary(garg_ctx* ctx, garg_stream* stream, double spin, double
                          uint32_t azimuths, uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap,
                          double* edge_out, double* xi_out) {
     REQUIRE(ctx && edge_out, "ctx and edge_out are required.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_shadow_boundary");
     REQUIRE(spin >= 0.0 && spin < 1.0, "spin must be in [0, 1).");
     REQUIRE(r_obs > 5.0, "the observer must sit outside r = 5.");
     REQUIRE(inclination_deg > 0.0 && inclination_deg < 180.0, "the inclination must be in (0, 180) degrees.");
@@ -1037,3 +1086,58 @@ int garg_shadow_boundary(garg_ctx* ctx, garg_stream* stream, double spin, double
 }
 
 }  // extern "C"
+
+// ---- graphs ------------------------------------------------------------------
+
+struct garg_graph {
+    garg_ctx* ctx;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int garg_graph_begin(garg_ctx* ctx, garg_stream* stream) {
+    REQUIRE(ctx, "ctx is required.");
+    hipStream_t s = stream_of(ctx, stream);
+    if (capturing(s)) return fail(GARG_E_STATE, "the stream is already capturing a graph.");
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "beginning a graph capture");
+    return GARG_OK;
+}
+
+int garg_graph_end(garg_ctx* ctx, garg_stream* stream, garg_graph** out) {
+    REQUIRE(ctx && out, "ctx and out are required.");
+    hipStream_t s = stream_of(ctx, stream);
+    if (!capturing(s)) return fail(GARG_E_STATE, "the stream is not capturing a graph.");
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamEndCapture(s, &g), "ending the graph capture");
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(GARG_E_DEVICE, "instantiating the graph: %s", hipGetErrorString(e));
+    }
+    *out = new garg_graph{ctx, g, exec};
+    return GARG_OK;
+}
+
+int garg_graph_launch(garg_graph* graph, garg_stream* stream) {
+    REQUIRE(graph, "graph is required.");
+    HIP_TRY(hipGraphLaunch(graph->exec, stream_of(graph->ctx, stream)), "launching the graph");
+    return GARG_OK;
+}
+
+int garg_graph_destroy(garg_graph* graph) {
+    if (!graph) return GARG_OK;
+    (void)hipGraphExecDestroy(graph->exec);
+    (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    return GARG_OK;
+}
+
+int garg_particles_dispatches(garg_particles* p, uint32_t* out) {
+    REQUIRE(p && out, "particles and out are required.");
+    REFUSE_WHILE_CAPTURING(p->ctx, (garg_stream*)nullptr, "garg_particles_dispatches");
+    HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
+    HIP_TRY(hipMemcpy(out, p->view.dispatches, 4, hipMemcpyDeviceToHost), "reading the dispatch count");
+    return GARG_OK;
+}
+

@@ -75,6 +75,7 @@ struct ParticleView {
     float* vz;
     float* temperature;
     uint32_t* respawns;
+    uint32_t* dispatches;  // one word: sim dispatches since seeding
     uint32_t count;
 };
 
@@ -104,6 +105,7 @@ struct SplatArgs {
     float peak_temperature, emission_scale;
     uint32_t exact, frame;
     uint32_t seed_lo, seed_hi;
+    const uint32_t* frame_counter;  // non-null: the frame is this word (read at run time)
 };
 
 struct ResolveArgs {

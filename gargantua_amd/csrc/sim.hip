@@ -81,6 +81,7 @@ GARG_DEV Spawned spawn(const SimArgs& a, uint32_t index, uint32_t event) {
 
 __global__ __launch_bounds__(256) void seed_kernel(SimArgs a) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i == 0u) a.p.dispatches[0] = 0u;
     if (i >= a.p.count) return;
     Spawned s = spawn(a, i, 0u);
     a.p.px[i] = s.px;
@@ -97,6 +98,8 @@ __global__ __launch_bounds__(256) void seed_kernel(SimArgs a) {
 // the plunge speed limit respawn (sim.metal:44-79).
 __global__ __launch_bounds__(256) void step_kernel(SimArgs a) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    // One thread counts the dispatch; nothing in this launch reads the count.
+    if (i == 0u) a.p.dispatches[0] += 1u;
     if (i >= a.p.count) return;
     float x = a.p.px[i], y = a.p.py[i], z = a.p.pz[i];
     float vx = a.p.vx[i], vy = a.p.vy[i], vz = a.p.vz[i];

@@ -88,7 +88,8 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs a) {
         }
     } else {
         // One corner with the trilinear weights as probabilities: unbiased.
-        uint4 h = pcg4d(make_uint4(i, a.frame, a.seed_lo, a.seed_hi));
+        const uint32_t frame = a.frame_counter ? *a.frame_counter : a.frame;
+        uint4 h = pcg4d(make_uint4(i, frame, a.seed_lo, a.seed_hi));
         int cx = unit_float(h.x) < fraction[0] ? high[0] : low[0];
         int cy = unit_float(h.y) < fraction[1] ? high[1] : low[1];
         int cz = unit_float(h.z) < fraction[2] ? high[2] : low[2];

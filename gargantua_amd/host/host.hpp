@@ -45,6 +45,7 @@ struct Options {
     // serve: arrival rate, duration, scale-down of the workload for a bounded run
     double serve_rate = 20.0;
     double serve_seconds = 300.0;
+    double serve_deadline = 0.0;  // stop the loop after this wall time (0 = run until every request is done)
     std::string serve_report;
     uint64_t serve_seed = 1;
     int serve_sequence_frames = 240;

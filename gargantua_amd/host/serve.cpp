@@ -106,6 +106,17 @@ void run_serve(const Options& o) {
               "allocating a volume version");
     int newest = 0;
     uint64_t version_counter = 0;
+    // One captured graph per version: a sim dispatch, the stochastic splat
+    // (its frame is the dispatch count, so every replay draws a new one) and
+    // the resolve, replayed each tick instead of re-enqueueing three launches.
+    std::vector<garg_graph*> disk_frame(kVersions, nullptr);
+    for (int i = 0; i < kVersions; ++i) {
+        check(garg_graph_begin(s.ctx, nullptr), "capturing the disk frame");
+        check(garg_particles_step(s.particles, nullptr, 1), "capturing the disk frame");
+        check(garg_splat_bin(versions[i].volume, s.particles, nullptr, 0u, GARG_FRAME_AUTO), "capturing the disk frame");
+        check(garg_splat_resolve(versions[i].volume, nullptr, 1u), "capturing the disk frame");
+        check(garg_graph_end(s.ctx, nullptr, &disk_frame[i]), "capturing the disk frame");
+    }
     // Sweep tables per observer radius for Schwarzschild requests.
     struct TableSet {
         float radius;
@@ -164,10 +175,17 @@ void run_serve(const Options& o) {
     const auto start = Clock::now();
     auto now_s = [&] { return std::chrono::duration<double>(Clock::now() - start).count(); };
     size_t completed = 0;
-    uint32_t splat_frame = 0;
 
+    double next_progress = 20.0;
     while (completed + refused < all.size()) {
         double now = now_s();
+        if (o.serve_deadline > 0 && now > o.serve_deadline) break;  // report what finished
+        if (now >= next_progress) {
+            line("t=" + std::to_string((int)now) + " s: " + std::to_string(completed) + " done, " +
+                 std::to_string(active.size()) + " on the GPU, " + std::to_string(pending.size()) + " waiting, " +
+                 std::to_string(all.size() - next_arrival) + " yet to arrive");
+            next_progress += 20.0;
+        }
         while (next_arrival < all.size() && all[next_arrival].arrival <= now) pending.push_back((int)next_arrival++);
         if (pending.empty() && active.empty()) {
             // Idle until the next arrival.
@@ -205,9 +223,7 @@ void run_serve(const Options& o) {
             }
         }
         if (slot >= 0) {
-            check(garg_particles_step(s.particles, nullptr, 1), "stepping the disk");
-            check(garg_splat_bin(versions[slot].volume, s.particles, nullptr, 0u, splat_frame++), "splatting");
-            check(garg_splat_resolve(versions[slot].volume, nullptr, 1u), "resolving");
+            check(garg_graph_launch(disk_frame[slot], nullptr), "advancing the disk");
             versions[slot].version = ++version_counter;
             newest = slot;
         }
@@ -318,23 +334,28 @@ void run_serve(const Options& o) {
     double wall = now_s();
     std::vector<double> latency;
     uint64_t exhausted = 0, truncated = 0;
-    for (auto& r : all)
-        if (r.admitted >= 0) {
+    size_t unfinished = 0;
+    for (auto& r : all) {
+        if (r.done < 0) ++unfinished;  // past the deadline: still queued or on the GPU
+        if (r.admitted >= 0 && r.done >= 0) {
             latency.push_back(r.done - r.arrival);
             exhausted += r.exhausted;
             truncated += r.truncated;
         }
+    }
     std::sort(latency.begin(), latency.end());
     auto pct = [&](double p) {
         return latency.empty() ? 0.0 : latency[std::min(latency.size() - 1, (size_t)((latency.size() - 1) * p))];
     };
     char buf[1024];
     snprintf(buf, sizeof buf,
-             "{\"requests\": %zu, \"completed\": %zu, \"refused\": %llu, \"wall_s\": %.3f, \"ticks\": %llu, "
+             "{\"requests\": %zu, \"completed\": %zu, \"unfinished\": %zu, \"refused\": %llu, \"wall_s\": %.3f, "
+             "\"ticks\": %llu, "
              "\"latency_p50_s\": %.4f, \"latency_p99_s\": %.4f, \"latency_max_s\": %.4f, \"steps\": %llu, "
              "\"steps_per_s\": %.4e, \"gpu_busy_fraction\": %.3f, \"exhausted_rays\": %llu, \"truncated_rays\": %llu, "
              "\"verified\": %llu, \"mismatched\": %llu, \"rate\": %.3f, \"arrival_seconds\": %.3f, \"frames\": %d}",
-             all.size(), completed, (unsigned long long)refused, wall, (unsigned long long)ticks, pct(0.5), pct(0.99),
+             all.size(), completed, unfinished, (unsigned long long)refused, wall, (unsigned long long)ticks, pct(0.5),
+             pct(0.99),
              latency.empty() ? 0.0 : latency.back(), (unsigned long long)total_steps, total_steps / wall,
              gpu_busy / wall, (unsigned long long)exhausted, (unsigned long long)truncated,
              (unsigned long long)verified, (unsigned long long)mismatched, o.serve_rate, o.serve_seconds,
@@ -348,6 +369,7 @@ void run_serve(const Options& o) {
         }
     }
     for (auto& t : tables) garg_tables_destroy(t.t);
+    for (garg_graph* g : disk_frame) garg_graph_destroy(g);
     for (int i = 1; i < kVersions; ++i) garg_volume_destroy(versions[i].volume);
     s.close();
     exit(mismatched == 0 ? 0 : 1);

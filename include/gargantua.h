@@ -60,6 +60,7 @@ typedef struct garg_volume garg_volume;
 typedef struct garg_tables garg_tables;
 typedef struct garg_targets garg_targets;
 typedef struct garg_bake garg_bake;
+typedef struct garg_graph garg_graph;
 
 /* ---- shared records (ShaderTypes.h) -------------------------------------- */
 
@@ -187,6 +188,9 @@ int garg_particles_read(garg_particles* particles, garg_particle* host, uint32_t
 int garg_particles_write(garg_particles* particles, const garg_particle* host, uint32_t count);
 /* 64-bit FNV-1a over the 32-byte AoS records (Validate.swift:378-387). */
 int garg_particles_checksum(garg_particles* particles, uint64_t* fnv1a);
+/* Sim dispatches since the last seed (each garg_particles_step dispatch adds
+ * one, on the device). */
+int garg_particles_dispatches(garg_particles* particles, uint32_t* out);
 
 /* ---- volume: Gpu/Volume.swift:14-43, SplatPass.swift:34-115 -------------- */
 
@@ -196,7 +200,12 @@ int garg_volume_destroy(garg_volume* volume);
 /* Volume.encodeClear: zero the fixed point sums. */
 int garg_volume_clear(garg_volume* volume, garg_stream* stream);
 /* splatParticles (splat.metal:123-167): exact = 1 deposits into all eight
- * corners, 0 picks one corner per particle from pcg4d(i, frame, seed). */
+ * corners, 0 picks one corner per particle from pcg4d(i, frame, seed).
+ * frame = GARG_FRAME_AUTO takes the frame from the particle store's dispatch
+ * count when the kernel runs, so a captured graph of step + splat + resolve
+ * draws a new frame on every replay (the reference passes a per-call
+ * counter, SplatPass.swift:18). */
+#define GARG_FRAME_AUTO 0xffffffffu
 int garg_splat_bin(garg_volume* volume, garg_particles* particles, garg_stream* stream, uint32_t exact, uint32_t frame);
 /* resolveVolume (splat.metal:172-199); clear_sums = 1 also zeroes the sums. */
 int garg_splat_resolve(garg_volume* volume, garg_stream* stream, uint32_t clear_sums);
@@ -317,6 +326,23 @@ int garg_deflection_table(garg_ctx* ctx, garg_stream* stream, double r_obs, uint
 int garg_shadow_boundary(garg_ctx* ctx, garg_stream* stream, double spin, double r_obs, double inclination_deg,
                          uint32_t azimuths, uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap,
                          double* edge_out, double* xi_out);
+
+/* ---- graphs --------------------------------------------------------------- */
+
+/* HIP graph capture of a launch sequence, the replacement for re-encoding
+ * the same command buffer every frame (Renderer.swift:119-215 encodes sim,
+ * splat and resolve per frame). Between begin and end every launch entry
+ * point enqueued on `stream` is recorded instead of run; entry points that
+ * copy to or from the host or wait for the device (reads, checksums, table
+ * updates, probes, the batched march, tonemap readback, syncs, destroys)
+ * refuse with GARG_E_STATE while the stream or the context's stream
+ * captures. No entry point records a memset node: clears are kernels.
+ * A replay runs the recorded launches with the arguments they had at
+ * capture (use GARG_FRAME_AUTO for a splat frame that advances). */
+int garg_graph_begin(garg_ctx* ctx, garg_stream* stream);
+int garg_graph_end(garg_ctx* ctx, garg_stream* stream, garg_graph** out);
+int garg_graph_launch(garg_graph* graph, garg_stream* stream);
+int garg_graph_destroy(garg_graph* graph);
 
 #ifdef __cplusplus
 }
