@@ -364,23 +364,22 @@ class Event:
             self.h = None
 
 
-class Stream:
-    """A HIP stream of the context; pass it wherever an entry point takes `stream`."""
+class Stream(C.c_void_p):
+    """A HIP stream of the context. The object is the garg_stream handle
+    itself, so it passes wherever an entry point takes `stream`."""
 
     def __init__(self, ctx: Context):
+        super().__init__()
         self.ctx = ctx
-        h = vp()
-        _check(ctx.lib.garg_stream_create(ctx.h, C.byref(h)))
-        self.h = h
-        self._as_param_ = h
+        _check(ctx.lib.garg_stream_create(ctx.h, C.byref(self)))
 
     def sync(self):
-        _check(self.ctx.lib.garg_stream_sync(self.ctx.h, self.h))
+        _check(self.ctx.lib.garg_stream_sync(self.ctx.h, self))
 
     def close(self):
-        if self.h:
-            self.ctx.lib.garg_stream_destroy(self.h)
-            self.h = None
+        if self.value:
+            self.ctx.lib.garg_stream_destroy(self)
+            self.value = None
 
 
 class Graph:

@@ -35,7 +35,7 @@ const char* kUsage =
     "usage: gargantua [--preset small|default|max] [--particles N] [--volume 96|128|192|256] [--seed S] "
     "[--strategy march|bake] [--fps-cap 30|60] [--no-upscale] [--spin A] [--camera az,el,dist] "
     "[--still out.png --width W --height H [--sequence N]] | bench [--sweep] | validate | serve [--rate R] "
-    "[--seconds T] [--deadline D] [--frames F] [--verify N] [--report PATH] | version | constants";
+    "[--seconds T] [--deadline D] [--frames F] [--tick-rays N] [--verify N] [--report PATH] | version | constants";
 
 namespace {
 
@@ -167,6 +167,9 @@ std::string parse_options(const std::vector<std::string>& args, Options& o) {
         } else if (a == "--deadline") {
             if (!(err = value(a, text)).empty()) return err;
             if (!parse_double(text, o.serve_deadline) || o.serve_deadline <= 0) return "deadline needs a positive number.";
+        } else if (a == "--tick-rays") {
+            if (!(err = integer(a, o.serve_tick_rays)).empty()) return err;
+            if (o.serve_tick_rays < 64) return "tick-rays needs at least 64 rays.";
         } else if (a == "--frames") {
             if (!(err = integer(a, o.serve_sequence_frames)).empty()) return err;
         } else if (a == "--verify") {

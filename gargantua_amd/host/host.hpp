@@ -46,6 +46,7 @@ struct Options {
     double serve_rate = 20.0;
     double serve_seconds = 300.0;
     double serve_deadline = 0.0;  // stop the loop after this wall time (0 = run until every request is done)
+    int serve_tick_rays = 4 << 20;  // rays marched per tick across all requests
     std::string serve_report;
     uint64_t serve_seed = 1;
     int serve_sequence_frames = 240;

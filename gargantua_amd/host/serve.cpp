@@ -56,6 +56,7 @@ struct Request {
     // runtime
     double admitted = -1, done = -1;
     int frame = 0;
+    int frame_slices = 0;  // batched launches the current frame's rays were split over
     uint32_t rays = 0, issued = 0, tick_issued = 0;
     int volume_slot = -1;
     uint64_t footprint = 0;
@@ -166,7 +167,7 @@ void run_serve(const Options& o) {
 
     garg_batch_config cfg;
     check(garg_batch_config_default(s.ctx, &cfg), "reading the batch configuration");
-    const uint64_t kTickRays = 4u << 20;  // rays marched per tick across all requests
+    const uint64_t kTickRays = (uint64_t)o.serve_tick_rays;  // rays marched per tick across all requests
     size_t next_arrival = 0;
     std::deque<int> pending;
     std::vector<int> active;
@@ -230,9 +231,11 @@ void run_serve(const Options& o) {
         // One batched launch over a slice of every active request's current frame.
         std::vector<garg_march_request> batch;
         uint64_t budget = kTickRays;
+        // Every active request starts the tick with nothing issued, also those
+        // the tick's ray budget does not reach.
+        for (int id : active) all[id].tick_issued = 0;
         for (int id : active) {
             Request& r = all[id];
-            r.tick_issued = 0;
             if (r.bake) continue;
             if (budget == 0) break;
             uint32_t take = (uint32_t)std::min<uint64_t>(budget, r.rays - r.issued);
@@ -248,6 +251,7 @@ void run_serve(const Options& o) {
             q.ray_begin = r.issued;
             q.ray_count = take;
             batch.push_back(q);
+            r.frame_slices += 1;
             r.tick_issued = take;
             budget -= take;
         }
@@ -295,6 +299,7 @@ void run_serve(const Options& o) {
             }
             if (++r.frame < r.frames) {
                 r.issued = 0;
+                r.frame_slices = 0;
                 r.uniforms = frame_uniforms(r);
                 still_active.push_back(id);
                 continue;
@@ -318,7 +323,21 @@ void run_serve(const Options& o) {
                 check(garg_targets_read(t2, alone.data(), alone.data() + (size_t)r.w * r.h * 4, nullptr), "reading");
                 garg_targets_destroy(t2);
                 ++verified;
-                if (fnv(mixed.data(), mixed.size() * 2) != fnv(alone.data(), alone.size() * 2)) ++mismatched;
+                if (fnv(mixed.data(), mixed.size() * 2) != fnv(alone.data(), alone.size() * 2)) {
+                    ++mismatched;
+                    size_t differing = 0, first = 0;
+                    for (size_t k = 0; k < (size_t)r.w * r.h; ++k)
+                        if (memcmp(&mixed[4 * k], &alone[4 * k], 8) != 0 ||
+                            memcmp(&mixed[(size_t)r.w * r.h * 4 + 4 * k], &alone[(size_t)r.w * r.h * 4 + 4 * k], 8) != 0)
+                            if (differing++ == 0) first = k;
+                    char msg[512];
+                    snprintf(msg, sizeof msg,
+                             "mismatch: request %d %dx%d spin %.3f budget %u frames %d slices %d slot %d: %zu pixels "
+                             "differ, first (%zu, %zu)",
+                             r.id, r.w, r.h, r.spin, r.budget, r.frames, r.frame_slices, r.volume_slot, differing,
+                             first % (size_t)r.w, first / (size_t)r.w);
+                    line(msg);
+                }
             }
             r.done = now_s();
             versions[r.volume_slot].pins -= 1;

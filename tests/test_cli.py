@@ -81,10 +81,14 @@ def test_validate_passes_every_judged_golden():
 
 
 @pytest.mark.gpu
-def test_serve_results_equal_alone_and_in_the_mix(tmp_path):
+@pytest.mark.parametrize("tick_rays", [4 << 20, 50_000])
+def test_serve_results_equal_alone_and_in_the_mix(tmp_path, tick_rays):
+    """Every request re-rendered alone equals its render in the mix; a small
+    per-tick ray budget splits frames over many ticks and leaves requests
+    waiting for budget."""
     report = tmp_path / "serve.json"
     r = run("serve", "--preset", "small", "--rate", "6", "--seconds", "3", "--frames", "3", "--verify", "1",
-            "--report", str(report))
+            "--tick-rays", str(tick_rays), "--report", str(report))
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     rep = json.loads(report.read_text())

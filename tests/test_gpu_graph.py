@@ -72,6 +72,9 @@ def test_captured_disk_frame_replays_like_eager_launches(ctx):
 
 
 def test_host_synchronous_calls_refuse_while_capturing(ctx):
+    """A call that waits for the device or copies to the host refuses with
+    GARG_E_STATE when its stream (or the context's stream) is capturing;
+    work on other streams is unaffected."""
     p, v = _disk(ctx, 10_000)
     s = ctx.stream()
     ctx.sync()
@@ -79,18 +82,24 @@ def test_host_synchronous_calls_refuse_while_capturing(ctx):
     try:
         p.step(1, stream=s)
         with pytest.raises(api.GargError) as e:
-            p.checksum()
-        assert e.value.status == api.GARG_E_STATE
-        with pytest.raises(api.GargError) as e:
             ctx.sync(s)
         assert e.value.status == api.GARG_E_STATE
+        before = p.checksum()  # on the context's stream, which is not capturing
     finally:
         g = ctx.graph_end(s)
-    assert p.dispatches() == 20
+    ctx.graph_begin()  # now the context's own stream captures
+    try:
+        p.step(1)
+        with pytest.raises(api.GargError) as e:
+            p.checksum()
+        assert e.value.status == api.GARG_E_STATE
+    finally:
+        g2 = ctx.graph_end()
+    assert p.dispatches() == 20 and p.checksum() == before
     g.launch(s)
     s.sync()
-    assert p.dispatches() == 21
-    g.close()
-    s.close()
-    v.close()
-    p.close()
+    g2.launch()
+    ctx.sync()
+    assert p.dispatches() == 22
+    for o in (g, g2, s, v, p):
+        o.close()
