@@ -592,9 +592,11 @@ GARG_DEV f3 kerr_direction_fast(float r, float st, float ct, float phi, float rd
     return len2 > 0.0f ? local * __builtin_amdgcn_rsqf(len2) : r_hat;
 }
 
-GARG_DEV KerrEval kerr_eval(const KerrConsts& k, float r, float th, float pr, float pth) {
+// The right-hand sides at a state whose sin and cos of theta are given.
+GARG_DEV KerrEval kerr_eval_sc(const KerrConsts& k, float r, float st, float ct, float pr, float pth) {
     KerrEval e;
-    fast_sincos(th, &e.st, &e.ct);
+    e.st = st;
+    e.ct = ct;
     float r2 = r * r;
     float delta = (r2 - 2.0f * r) + k.a2;
     float sigma = r2 + k.a2 * (e.ct * e.ct);
@@ -628,6 +630,25 @@ GARG_DEV KerrEval kerr_eval(const KerrConsts& k, float r, float th, float pr, fl
     e.inv_sigma = inv_sigma;
     return e;
 }
+
+GARG_DEV KerrEval kerr_eval(const KerrConsts& k, float r, float th, float pr, float pth) {
+    float st, ct;
+    fast_sincos(th, &st, &ct);
+    return kerr_eval_sc(k, r, st, ct, pr, pth);
+}
+
+// A Runge-Kutta stage at theta + d from the evaluation `at` of theta: the
+// stage's sin and cos come from `at`'s by angle addition (sincos_shift).
+GARG_DEV KerrEval kerr_eval_stage(const KerrConsts& k, const KerrEval& at, float r, float th, float d, float pr,
+                                  float pth) {
+    float st, ct;
+    sincos_shift(at.st, at.ct, d, th + d, &st, &ct);
+    return kerr_eval_sc(k, r, st, ct, pr, pth);
+}
+
+struct NoTick {
+    GARG_DEV void operator()(uint32_t) const {}
+};
 
 // What the fast Kerr loop reads besides the ray: a by-value block, so a
 // caller whose wave shares one request can hold it in scalar registers.
@@ -694,12 +715,15 @@ struct KerrFastMarch {
 
     // Advance by at most `budget` steps. UNIFORM: every lane of the wave
     // passes the same `l`, which is then read from the first lane only.
-    template <bool UNIFORM>
-    GARG_DEV void run_loop(KerrLoop l, uint32_t budget) {
+    // `tick` runs once per step of the wave (scalar work only: it sees the
+    // wave, not the lane).
+    template <bool UNIFORM, class Tick = NoTick>
+    GARG_DEV void run_loop(KerrLoop l, uint32_t budget, Tick tick = Tick{}) {
         if constexpr (UNIFORM) l = wave_uniform(l);
         const bool has_volume = l.gather != 0u;
         const bool compensated = l.compensated != 0u;
         for (uint32_t n = 0; n < budget && !done; ++n) {
+            tick(n);
             if (steps >= l.step_cap) {
                 done = true;
                 break;
@@ -708,9 +732,9 @@ struct KerrFastMarch {
             float pole = fminf(fmaxf(e.st * (1.0f / GARG_F(KERR_POLE_SINE)), 0.05f), 1.0f);
             float h = fminf(fmaxf(fminf(l.step_factor * r, near_h) * pole, GARG_F(KERR_STEP_FLOOR)), l.step_max);
             float hh = 0.5f * h;
-            KerrEval k2 = kerr_eval(k, r + e.dr * hh, th + e.dth * hh, pr + e.dpr * hh, pth + e.dpth * hh);
-            KerrEval k3 = kerr_eval(k, r + k2.dr * hh, th + k2.dth * hh, pr + k2.dpr * hh, pth + k2.dpth * hh);
-            KerrEval k4 = kerr_eval(k, r + k3.dr * h, th + k3.dth * h, pr + k3.dpr * h, pth + k3.dpth * h);
+            KerrEval k2 = kerr_eval_stage(k, e, r + e.dr * hh, th, e.dth * hh, pr + e.dpr * hh, pth + e.dpth * hh);
+            KerrEval k3 = kerr_eval_stage(k, e, r + k2.dr * hh, th, k2.dth * hh, pr + k2.dpr * hh, pth + k2.dpth * hh);
+            KerrEval k4 = kerr_eval_stage(k, e, r + k3.dr * h, th, k3.dth * h, pr + k3.dpr * h, pth + k3.dpth * h);
             float w = h * (1.0f / 6.0f);
             float ir = ((e.dr + k2.dr * 2.0f) + (k3.dr * 2.0f + k4.dr)) * w;
             float ith = ((e.dth + k2.dth * 2.0f) + (k3.dth * 2.0f + k4.dth)) * w;

@@ -61,4 +61,31 @@ GARG_HD void fast_sincos(float x, float* s, float* c) {
     *c = ((q + 1) & 2) ? -cc : cc;
 }
 
+// sin and cos of x0 + d from s0 = sin x0 and c0 = cos x0 by the angle
+// addition formulas, with sin d and cos d from their Taylor series (to d^7
+// and d^6; the first omitted terms are below 1.1e-11 and 4e-10 for
+// |d| <= 1/4). Shifts beyond 1/4 fall back to fast_sincos(x), where x is
+// the caller's x0 + d. The Runge-Kutta stages of the fast integrator move
+// theta by such small shifts from a point whose sine and cosine are known.
+#define GARG_SHIFT_LIMIT 0.25f
+
+GARG_HD void sincos_shift(float s0, float c0, float d, float x, float* s, float* c) {
+    if (fabsf(d) <= GARG_SHIFT_LIMIT) {
+        float d2 = d * d;
+        float ps = fmaf(d2, -1.98412698412698413e-04f, 8.33333333333333333e-03f);
+        ps = fmaf(d2, ps, -1.66666666666666667e-01f);
+        ps = ps * d2;
+        float sd = fmaf(d, ps, d);
+        float pc = fmaf(d2, -1.38888888888888889e-03f, 4.16666666666666667e-02f);
+        pc = fmaf(d2, pc, -0.5f);
+        float cd = fmaf(d2, pc, 1.0f);
+        float cs = c0 * sd;
+        float ss = s0 * sd;
+        *s = fmaf(s0, cd, cs);
+        *c = fmaf(c0, cd, -ss);
+    } else {
+        fast_sincos(x, s, c);
+    }
+}
+
 }  // namespace garg

@@ -15,6 +15,9 @@
 namespace garg {
 
 constexpr int XCDS = 8;
+// Real-time age (100 MHz ticks, 200 us) at which a tile first rises in issue
+// priority; it rises again at twice and four times that age.
+constexpr uint64_t TILE_AGE_STEP = 20000;
 constexpr int QUEUE_STRIDE = 16;  // 128 bytes between queue heads
 
 GARG_DEV uint64_t wave_broadcast(uint64_t v) {
@@ -69,6 +72,23 @@ GARG_DEV void finish_ray(const BatchRequest& r, CounterTable& t, uint32_t reques
     if (steps) atomicAdd(&t.steps[request], (unsigned long long)steps);
 }
 
+// Raises the wave's issue priority as its tile ages: one level at
+// TILE_AGE_STEP, two at twice that, three at four times (checked every 8
+// steps from the 100 MHz real-time counter).
+struct TileAging {
+    uint64_t born;
+    uint32_t priority = 0;
+    GARG_DEV void operator()(uint32_t n) {
+        if ((n & 7u) != 7u || priority >= 3u) return;
+        const uint64_t age = __builtin_amdgcn_s_memrealtime() - born;
+        if (age <= (TILE_AGE_STEP << priority)) return;
+        ++priority;
+        if (priority == 1u) __builtin_amdgcn_s_setprio(1);
+        else if (priority == 2u) __builtin_amdgcn_s_setprio(2);
+        else __builtin_amdgcn_s_setprio(3);
+    }
+};
+
 template <bool KERR>
 struct Lane;
 
@@ -81,9 +101,9 @@ struct Lane<true> {
         m.begin(u.spin, kerr_launch(u.spin, load3(u.camera_position), dir));
         return true;
     }
-    template <bool UNIFORM>
-    GARG_DEV void run(const BatchRequest& r, uint32_t budget) {
-        m.template run_loop<UNIFORM>(kerr_loop(r.uniforms, r.scene, true, r.uniforms.compensated != 0u), budget);
+    template <bool UNIFORM, class Tick = NoTick>
+    GARG_DEV void run(const BatchRequest& r, uint32_t budget, Tick tick = Tick{}) {
+        m.template run_loop<UNIFORM>(kerr_loop(r.uniforms, r.scene, true, r.uniforms.compensated != 0u), budget, tick);
     }
     GARG_DEV bool done() const { return m.done; }
     GARG_DEV void finish(const BatchRequest& r, CounterTable& t, uint32_t request, uint32_t pixel) {
@@ -111,8 +131,8 @@ struct Lane<false> {
         m.begin(p.ray);
         return true;
     }
-    template <bool UNIFORM>
-    GARG_DEV void run(const BatchRequest& r, uint32_t budget) {
+    template <bool UNIFORM, class Tick = NoTick>
+    GARG_DEV void run(const BatchRequest& r, uint32_t budget, Tick = Tick{}) {
         m.run(r.uniforms, r.scene, true, r.uniforms.compensated != 0u, budget);
     }
     GARG_DEV bool done() const { return m.done; }
@@ -278,10 +298,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
         bool live = lane < got && ray_pixel(req, base + lane, px, py);
         const uint32_t pixel = py * req.uniforms.resolution[0] + px;
         if (live) live = ray.start(req, px, py, table, r, pixel);
-        if (live) {
-            ray.template run<true>(req, 0xffffffffu);
-            ray.finish(req, table, r, pixel);
-        }
+        // Issue arbitration among the waves of a SIMD favours the same
+        // hardware wave slots for as long as the waves live; a persistent
+        // wave in a losing slot would keep one tile for the whole frame.
+        // A tile therefore starts at the lowest priority and rises as it
+        // ages in real time, so the oldest tiles finish first, as they do
+        // when the hardware launches a fresh wave per tile.
+        __builtin_amdgcn_s_setprio(0);
+        if (live) ray.template run<true>(req, 0xffffffffu, TileAging{__builtin_amdgcn_s_memrealtime()});
+        if (live) ray.finish(req, table, r, pixel);
         // Claiming only now, not before the march, keeps the claim's result
         // out of the registers the march needs.
         uint64_t next = 0;

@@ -56,3 +56,45 @@ def test_fast_sincos_accuracy():
     worst_s, worst_c, worst_abs, worst_small = (float(v) for v in out[:4])
     print(f"sin {worst_s:.2f} ulp, cos {worst_c:.2f} ulp, abs {worst_abs:.2e}, small-angle rel {worst_small:.2e}")
     assert worst_s < 4 and worst_c < 4 and worst_abs < 3e-7 and worst_small < 2e-7
+
+
+SHIFT = r"""
+#include <math.h>
+#include <stdio.h>
+#include "fast_math.hpp"
+int main(void) {
+    double worst = 0, worst_fallback = 0;
+    for (int i = 0; i <= 4000; ++i) {
+        float x0 = -1.0f + 5.0f * (float)i / 4000.0f;   /* theta near [0, pi] and past the poles */
+        float s0, c0;
+        garg::fast_sincos(x0, &s0, &c0);
+        for (int j = -500; j <= 500; ++j) {
+            float d = 0.3f * (float)j / 500.0f;         /* inside and past the series limit */
+            float x = x0 + d;
+            float s, c;
+            garg::sincos_shift(s0, c0, d, x, &s, &c);
+            double want_s = sin((double)x0 + (double)d), want_c = cos((double)x0 + (double)d);
+            double e = fmax(fabs(s - want_s), fabs(c - want_c));
+            if (fabsf(d) <= GARG_SHIFT_LIMIT) worst = fmax(worst, e);
+            else worst_fallback = fmax(worst_fallback, fmax(fabs(s - sin((double)x)), fabs(c - cos((double)x))));
+        }
+    }
+    printf("%g %g\n", worst, worst_fallback);
+    return 0;
+}
+"""
+
+
+def test_sincos_shift_accuracy():
+    """Angle addition from a known sine and cosine: absolute error within a
+    few float ulp of 1 over the shift range, fast_sincos beyond it."""
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "s.cpp"), os.path.join(d, "s")
+        open(src, "w").write(SHIFT)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+                        os.path.join(ROOT, "gargantua_amd", "csrc"), src, "-o", exe, "-lm"], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    worst, fallback = float(out[0]), float(out[1])
+    print(f"shifted sincos abs error {worst:.2e}, fallback {fallback:.2e}")
+    assert worst < 4e-7 and fallback < 3e-7
+
