@@ -278,10 +278,18 @@ def pass_leg(ctx, api, particles, vol):
     resolve_ms = timed(lambda: (vol.splat(particles, exact=True), vol.resolve(clear=True)), 10) - bin_ms
     sim_bytes = 64 * n
     resolve_bytes = (24 + 16 + 24) * voxels
+    # The ceiling for the bin: the same count of no-return atomics (8 corners
+    # x 6 sums per particle) to random words of the sums' footprint.
+    ceiling = ctx.atomic_rate(6 * voxels, 48, n)
     return {"sim": {"ms": sim_ms, "particles_per_s": n / (sim_ms * 1e-3), "bytes": sim_bytes,
                     "GB_per_s": sim_bytes / (sim_ms * 1e-3) / 1e9, "hbm_frac": sim_bytes / (sim_ms * 1e-3) / 8e12},
+            # Exact bin: 8 corners x 6 integer sums per particle. Sorted and
+            # aggregated in LDS, it issues far fewer global atomics than that,
+            # so its corner-sum rate is compared with what direct atomics reach.
             "splat_bin_exact": {"ms": bin_ms, "deposits_per_s": n / (bin_ms * 1e-3),
-                                "corner_atomics_per_s": 8 * 6 * n / (bin_ms * 1e-3)},
+                                "corner_sums_per_s": 8 * 6 * n / (bin_ms * 1e-3),
+                                "direct_atomic_ceiling_per_s": ceiling,
+                                "vs_direct_atomic_ceiling": 8 * 6 * n / (bin_ms * 1e-3) / ceiling},
             "resolve": {"ms": resolve_ms, "voxels_per_s": voxels / (resolve_ms * 1e-3), "bytes": resolve_bytes,
                         "GB_per_s": resolve_bytes / (resolve_ms * 1e-3) / 1e9,
                         "hbm_frac": resolve_bytes / (resolve_ms * 1e-3) / 8e12}}

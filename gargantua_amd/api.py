@@ -144,6 +144,7 @@ SIGNATURES = {
     "garg_deflection_table": (C.c_int, [vp, vp, f64, u32, f64, f64, u32, vp, vp, vp]),
     "garg_shadow_boundary": (C.c_int, [vp, vp, f64, f64, f64, u32, u32, u32, u32, vp, vp]),
     "garg_particles_dispatches": (C.c_int, [vp, P(u32)]),
+    "garg_atomic_rate": (C.c_int, [vp, vp, u32, u32, u32, P(f64)]),
     "garg_graph_begin": (C.c_int, [vp, vp]),
     "garg_graph_end": (C.c_int, [vp, vp, P(vp)]),
     "garg_graph_launch": (C.c_int, [vp, vp]),
@@ -276,6 +277,12 @@ class Context:
 
     def stream(self) -> "Stream":
         return Stream(self)
+
+    def atomic_rate(self, words: int, adds_per_thread: int, threads: int) -> float:
+        """No-return 32-bit atomic adds per second to random words (garg_atomic_rate)."""
+        r = f64()
+        _check(self.lib.garg_atomic_rate(self.h, None, words, adds_per_thread, threads, C.byref(r)))
+        return r.value
 
     def graph_begin(self, stream=None):
         """Record every launch enqueued on `stream` until graph_end."""

@@ -88,6 +88,9 @@ struct garg_particles {
     void* base;
     uint64_t bytes;
     ParticleView view;
+    void* sort_base;  // the sorted exact deposit's keys, order and sort storage
+    uint64_t sort_bytes;
+    SplatSortBuffers sort;
 };
 
 struct garg_volume {
@@ -381,6 +384,20 @@ int garg_particles_create(garg_ctx* ctx, uint32_t count, uint64_t seed, uint32_t
     // The dispatch count starts at zero even before the first seed.
     rc = launch_fill_zero(p->view.dispatches, 256, ctx->stream) == hipSuccess ? GARG_OK
                                                                                : fail(GARG_E_DEVICE, "clearing the dispatch count");
+    if (!rc) {
+        uint64_t words = align256((uint64_t)count * 4u);
+        p->sort.temp_bytes = splat_sort_temp_bytes(count);
+        p->sort_bytes = 4u * words + align256(p->sort.temp_bytes);
+        rc = ctx_alloc(ctx, p->sort_bytes, &p->sort_base, "the deposit's sort buffers");
+        if (!rc) {
+            char* s = static_cast<char*>(p->sort_base);
+            p->sort.keys_in = (uint32_t*)s;
+            p->sort.keys_out = (uint32_t*)(s + words);
+            p->sort.order_in = (uint32_t*)(s + 2 * words);
+            p->sort.order_out = (uint32_t*)(s + 3 * words);
+            p->sort.temp = s + 4 * words;
+        }
+    }
     if (rc) {
         ctx_free(ctx, p->base, p->bytes);
         delete p;
@@ -395,6 +412,7 @@ int garg_particles_destroy(garg_particles* p) {
     REFUSE_WHILE_CAPTURING(p->ctx, (garg_stream*)nullptr, "garg_particles_destroy");
     (void)hipStreamSynchronize(p->ctx->stream);
     ctx_free(p->ctx, p->base, p->bytes);
+    ctx_free(p->ctx, p->sort_base, p->sort_bytes);
     delete p;
     return GARG_OK;
 }
@@ -551,7 +569,12 @@ int garg_splat_bin(garg_volume* v, garg_particles* p, garg_stream* stream, uint3
     a.seed_lo = (uint32_t)p->seed;
     a.seed_hi = (uint32_t)(p->seed >> 32);
     a.frame_counter = frame == GARG_FRAME_AUTO ? p->view.dispatches : nullptr;
-    HIP_TRY(launch_splat_bin(a, stream_of(v->ctx, stream)), "launching splatParticles");
+    hipStream_t s = stream_of(v->ctx, stream);
+    // The exact deposit sorts and aggregates in LDS (the same integer sums,
+    // far fewer global atomics). Inside a graph capture it deposits
+    // directly: the library sort is not recorded into graphs.
+    if (a.exact && !capturing(s)) HIP_TRY(launch_splat_sorted(a, p->sort, s), "launching the sorted splat");
+    else HIP_TRY(launch_splat_bin(a, s), "launching splatParticles");
     return GARG_OK;
 }
 
@@ -1138,6 +1161,36 @@ int garg_particles_dispatches(garg_particles* p, uint32_t* out) {
     REFUSE_WHILE_CAPTURING(p->ctx, (garg_stream*)nullptr, "garg_particles_dispatches");
     HIP_TRY(hipDeviceSynchronize(), "waiting for the device");
     HIP_TRY(hipMemcpy(out, p->view.dispatches, 4, hipMemcpyDeviceToHost), "reading the dispatch count");
+    return GARG_OK;
+}
+
+// ---- measurement ---------------------------------------------------------------
+
+int garg_atomic_rate(garg_ctx* ctx, garg_stream* stream, uint32_t words, uint32_t adds_per_thread, uint32_t threads,
+                     double* adds_per_second) {
+    REQUIRE(ctx && adds_per_second, "ctx and adds_per_second are required.");
+    REQUIRE(words > 0 && adds_per_thread > 0 && threads > 0, "words, adds and threads must be positive.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_atomic_rate");
+    hipStream_t s = stream_of(ctx, stream);
+    void* buffer = nullptr;
+    int rc = ctx_alloc(ctx, (uint64_t)words * 4u, &buffer, "the atomic buffer");
+    if (rc) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = launch_fill_zero(buffer, (uint64_t)words * 4u, s);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = launch_atomic_rate((uint32_t*)buffer, words, adds_per_thread, threads, s);  // warm
+    if (e == hipSuccess) e = hipEventRecord(a, s);
+    if (e == hipSuccess) e = launch_atomic_rate((uint32_t*)buffer, words, adds_per_thread, threads, s);
+    if (e == hipSuccess) e = hipEventRecord(b, s);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    ctx_free(ctx, buffer, (uint64_t)words * 4u);
+    HIP_TRY(e, "timing the atomic kernel");
+    *adds_per_second = ms > 0.0f ? (double)threads * adds_per_thread / (ms * 1e-3) : 0.0;
     return GARG_OK;
 }
 

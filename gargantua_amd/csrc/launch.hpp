@@ -121,8 +121,22 @@ struct ResolveArgs {
 };
 
 hipError_t launch_splat_bin(const SplatArgs& a, hipStream_t stream);
+
+// Buffers of the sorted exact deposit: keys and particle order (in and out
+// of the radix sort, `count` words each) and the sort's temporary storage.
+struct SplatSortBuffers {
+    uint32_t* keys_in;
+    uint32_t* keys_out;
+    uint32_t* order_in;
+    uint32_t* order_out;
+    void* temp;
+    size_t temp_bytes;
+};
+size_t splat_sort_temp_bytes(uint32_t count);
+hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, hipStream_t stream);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
 hipError_t launch_fill_zero(void* ptr, uint64_t bytes, hipStream_t stream);
+hipError_t launch_atomic_rate(uint32_t* buffer, uint32_t words, uint32_t adds, uint32_t threads, hipStream_t stream);
 
 // FP64 sweep tables: table[i] for the sweep from `radius` over b at table
 // coordinate i / (n - 1) between low and high; kind 0 = sweep to infinity,

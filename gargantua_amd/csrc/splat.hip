@@ -143,6 +143,25 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(uint4* p, uint64_t n16, 
     if (blockIdx.x == 0 && threadIdx.x < tail_bytes) tail[threadIdx.x] = 0u;
 }
 
+// The atomic ceiling the splat is measured against: every thread adds 1 to
+// `adds` words of a `words`-word buffer, chosen by pcg4d(thread, k) — the
+// same no-return 32-bit atomics as the deposit, spread over the footprint
+// of the sums with no more contention than chance gives.
+__global__ __launch_bounds__(256) void atomic_rate_kernel(uint32_t* buffer, uint32_t words, uint32_t adds,
+                                                          uint32_t threads) {
+    uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= threads) return;
+    for (uint32_t k = 0; k < adds; ++k) {
+        uint4 h = pcg4d(make_uint4(t, k, 0x9e3779b9u, 0x7f4a7c15u));
+        __hip_atomic_fetch_add(&buffer[h.x % words], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+hipError_t launch_atomic_rate(uint32_t* buffer, uint32_t words, uint32_t adds, uint32_t threads, hipStream_t stream) {
+    atomic_rate_kernel<<<dim3((threads + 255u) / 256u), 256, 0, stream>>>(buffer, words, adds, threads);
+    return hipGetLastError();
+}
+
 hipError_t launch_splat_bin(const SplatArgs& a, hipStream_t stream) {
     splat_kernel<<<dim3((a.p.count + 255u) / 256u), 256, 0, stream>>>(a);
     return hipGetLastError();

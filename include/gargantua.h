@@ -327,6 +327,15 @@ int garg_shadow_boundary(garg_ctx* ctx, garg_stream* stream, double spin, double
                          uint32_t azimuths, uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap,
                          double* edge_out, double* xi_out);
 
+/* ---- measurement ------------------------------------------------------------ */
+
+/* No counterpart in the reference: the ceiling the splat bin is measured
+ * against. Times `threads` x `adds_per_thread` no-return 32-bit atomic adds
+ * to pcg4d-chosen words of a `words`-word buffer (allocated and freed here)
+ * and returns the rate. Synchronous. */
+int garg_atomic_rate(garg_ctx* ctx, garg_stream* stream, uint32_t words, uint32_t adds_per_thread, uint32_t threads,
+                     double* adds_per_second);
+
 /* ---- graphs --------------------------------------------------------------- */
 
 /* HIP graph capture of a launch sequence, the replacement for re-encoding

@@ -98,8 +98,10 @@ def test_seed_matches_oracle_spawn(ctx):
 
 
 @pytest.mark.parametrize("exact", [True, False])
-@pytest.mark.parametrize("size,hz", [(96, 30.0), (128, 3.0)])
+@pytest.mark.parametrize("size,hz", [(96, 30.0), (128, 3.0), (256, 3.0), (12, 30.0)])
 def test_splat_bit_exact_against_cpu_replay(ctx, exact, size, hz):
+    """The exact deposit sorts particles and aggregates in LDS; a 12^3 grid
+    makes most runs cross z planes, which deposit directly."""
     p = settled(ctx, 250_000, 3, 100)
     v = api.Volume(ctx, size, 30.0, hz)
     v.splat(p, exact=exact, frame=17)
@@ -150,3 +152,31 @@ def test_resolve_against_float64_restatement(ctx):
     assert np.allclose(velocity.reshape(-1, 4)[:, :3], want_v, rtol=2e-3, atol=2e-3)
     v.close()
     p.close()
+
+
+def test_sorted_exact_deposit_equals_direct_deposit(ctx):
+    """Inside a graph capture the exact deposit takes the direct path (no
+    library sort in graphs); eagerly it sorts and aggregates in LDS. Both
+    must give the same integer sums."""
+    p = settled(ctx, 300_000, 5, 60)
+    eager = api.Volume(ctx, 128, 30.0, 3.0)
+    graphed = api.Volume(ctx, 128, 30.0, 3.0)
+    s = ctx.stream()
+    ctx.sync()
+    ctx.graph_begin(s)
+    try:
+        graphed.splat(p, exact=True, stream=s)
+    finally:
+        g = ctx.graph_end(s)
+    for _ in range(2):  # each replay adds the same deposit again
+        g.launch(s)
+        s.sync()
+        eager.splat(p, exact=True)
+        ctx.sync()
+    e1, m1, a1 = eager.read_sums()
+    e2, m2, a2 = graphed.read_sums()
+    assert a1.sum() > 0
+    assert np.array_equal(e1, e2) and np.array_equal(m1, m2) and np.array_equal(a1, a2)
+    for o in (g, s, eager, graphed, p):
+        o.close()
+
