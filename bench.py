@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--config", default="", help="batch config as block,waves,period,threshold")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 64-request mixed batch")
     ap.add_argument("--no-passes", action="store_true", help="skip the sim/splat/resolve and FP64 legs")
+    ap.add_argument("--no-schwarzschild", action="store_true", help="skip the spin-0 legs (small and default presets)")
     return ap.parse_args()
 
 
@@ -214,6 +215,8 @@ def main():
     if not args.no_passes:
         line["passes"] = pass_leg(ctx, api, p, vol)
         line["fp64"] = fp64_leg(ctx)
+    if not args.no_schwarzschild:
+        line["schwarzschild"] = schwarzschild_legs(ctx, api, vol, cfg, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         every = args.cpu_rows or 1
         line["cpu_baseline"] = cpu_baseline(u, vol, every)
@@ -223,6 +226,101 @@ def main():
     for o in (targets, vol, p):
         o.close()
     ctx.close()
+
+
+def schwarzschild_legs(ctx, api, default_volume, cfg, cpu: bool):
+    """The spin-0 presets with the far-field jump: small (250k particles
+    from the annulus generator, 1000 substeps, exact CIC into 96^3 with
+    |z| <= 3, 640x360, 256-step interactive budget, observer r = 50,
+    inclination 80 deg, 60 deg field of view) and default (the 1M / 128^3
+    volume above, 960x540 at 256 steps, then a 3840x2160 still at 4096 steps
+    with compensated accumulation). Each: the batched launch against the
+    plain one in both integrator forms; the small frame also on the FP64
+    oracle over every row."""
+    small_p = api.Particles(ctx, 250_000, seed=1)
+    small_p.seed(api.GENERATOR_ANNULUS, sigma_z=0.3)
+    small_p.step(500)
+    small_v = api.Volume(ctx, 96, 30.0, 3.0)
+    small_v.clear()
+    small_v.splat(small_p, exact=True)
+    small_v.resolve(clear=True)
+    ctx.sync()
+    tables = api.Tables(ctx)
+    tables.update(50.0)
+    cam = api.orbit_camera(azimuth=0.0, elevation=math.radians(10.0), distance=50.0, fov_y_deg=60.0)
+    out = {}
+    for name, w, h, kind, cap, vol in (("small_640x360_256", 640, 360, "interactive", 256, small_v),
+                                       ("default_960x540_256", 960, 540, "interactive", 256, default_volume),
+                                       ("default_3840x2160_4096", 3840, 2160, "still", 4096, default_volume)):
+        u = api.uniforms(cam, w, h, api.settings(kind, step_cap=cap), compensated=kind == "still", star_seed=1,
+                         drift_budget=1e-5 if kind == "still" else 1e-4, volume_half_extent=30.0)
+        tables.apply(u)
+        t = api.Targets(ctx, w, h)
+        req = [{"uniforms": u, "targets": t, "volume": vol, "tables": tables}]
+        t.reset_counters()
+        totals = ctx.march_batch(req, cfg)
+        steps = int(totals.steps)
+        reps = 3 if cap > 256 else 10
+        batched_ms, frames = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, req, cfg), 1, reps)
+        leg = {"rays": int(totals.rays), "steps": steps, "over_drift_budget": int(totals.over_drift_budget),
+               "exhausted": int(totals.exhausted), "batched_ms": batched_ms / reps,
+               "batched_steps_per_s": steps / (batched_ms / reps * 1e-3)}
+        for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
+            ms, _ = time_frames(ctx, api, lambda: ctx.march_image(u, t, vol, tables, integrator), 1, reps)
+            leg[form + "_ms"] = ms / reps
+        leg["speedup_vs_plain_reference"] = leg["plain_reference_ms"] / leg["batched_ms"]
+        if cpu and name.startswith("small"):
+            import oracle as O
+            from tests import scenes
+            field = scenes.field_from_volume(vol)
+            params = scenes.oracle_params(u, scenes.oracle_tables(50.0))
+            xs, ys = np.meshgrid(np.arange(w), np.arange(h))
+            xy = np.stack([xs.ravel(), ys.ravel()], 1)
+            threads = max(1, min(16, os.cpu_count() or 1))
+            t0 = time.perf_counter()
+            _, total = scenes.oracle_pixels(params, xy, field, threads)
+            dt = time.perf_counter() - t0
+            leg["cpu_oracle"] = {"steps_per_s": total / dt, "seconds": dt, "cores": threads, "steps": total}
+        out[name] = leg
+        t.close()
+    out["small_640x360_256_x8"] = small_batch_leg(ctx, api, small_v, tables, cfg)
+    tables.close()
+    small_v.close()
+    small_p.close()
+    return out
+
+
+def small_batch_leg(ctx, api, vol, tables, cfg, views=8):
+    """Eight small frames (640x360, 256 steps, the small preset's volume)
+    from eight azimuths around the hole: one batched launch against eight
+    plain launches. A single small frame is latency-bound (its slowest tiles
+    set the time); marching several together fills the machine."""
+    reqs = []
+    for k in range(views):
+        cam = api.orbit_camera(azimuth=2.0 * math.pi * k / views, elevation=math.radians(10.0), distance=50.0,
+                               fov_y_deg=60.0)
+        u = api.uniforms(cam, 640, 360, api.settings("interactive", step_cap=256), star_seed=1, drift_budget=1e-4,
+                         volume_half_extent=30.0)
+        tables.apply(u)
+        reqs.append({"uniforms": u, "targets": api.Targets(ctx, 640, 360), "volume": vol, "tables": tables})
+    for r in reqs:
+        r["targets"].reset_counters()
+    totals = ctx.march_batch(reqs, cfg)
+    steps = int(totals.steps)
+    batched_ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, reqs, cfg), 1, 10)
+    out = {"requests": views, "rays": int(totals.rays), "steps": steps, "batched_ms": batched_ms / 10,
+           "batched_steps_per_s": steps / (batched_ms / 10 * 1e-3)}
+    for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
+        def run_all():
+            for r in reqs:
+                ctx.march_image(r["uniforms"], r["targets"], vol, tables, integrator)
+        ms, _ = time_frames(ctx, api, run_all, 1, 10)
+        out[form + "_ms"] = ms / 10
+    out["speedup_vs_plain_reference"] = out["plain_reference_ms"] / out["batched_ms"]
+    out["speedup_vs_plain_fast"] = out["plain_fast_ms"] / out["batched_ms"]
+    for r in reqs:
+        r["targets"].close()
+    return out
 
 
 def mixed_leg(ctx, api, vol, args):
@@ -320,6 +418,7 @@ def _batch_nosync(ctx, api, request, cfg):
         arr[i].uniforms = r["uniforms"]
         arr[i].targets = r["targets"].h
         arr[i].volume = r["volume"].h if r.get("volume") else None
+        arr[i].tables = r["tables"].h if r.get("tables") else None
         arr[i].ray_begin = r.get("ray_begin", 0)
         arr[i].ray_count = r.get("ray_count", 0)
     st = api.load().garg_march_batch(ctx.h, None, arr, len(request), C.byref(cfg), None)

@@ -11,6 +11,12 @@
 
 namespace garg {
 
+// A per-step hook that does nothing.
+struct NoTick {
+    GARG_DEV void operator()(uint32_t) const {}
+};
+
+
 // ---------------------------------------------------------------------------
 // Schwarzschild (Geodesic.h:9-188). The Eddington-Finkelstein time v is never
 // read by anything the march outputs, so it is not carried.
@@ -226,10 +232,14 @@ struct SchwMarch {
     }
 
     // Advance by at most `budget` steps; `gather` false skips the volume.
-    GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget) {
+    // `tick` runs once per step of the wave (scalar work only).
+    template <class Tick = NoTick>
+    GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget,
+                      Tick tick = Tick{}) {
         const garg_geodesic_settings& gs = u.geodesic;
         const bool has_volume = gather && sc.volume.voxels != nullptr;
         for (uint32_t n = 0; n < budget && !done; ++n) {
+            tick(n);
             if (steps >= gs.step_cap) {
                 done = true;
                 break;
@@ -645,10 +655,6 @@ GARG_DEV KerrEval kerr_eval_stage(const KerrConsts& k, const KerrEval& at, float
     sincos_shift(at.st, at.ct, d, th + d, &st, &ct);
     return kerr_eval_sc(k, r, st, ct, pr, pth);
 }
-
-struct NoTick {
-    GARG_DEV void operator()(uint32_t) const {}
-};
 
 // What the fast Kerr loop reads besides the ray: a by-value block, so a
 // caller whose wave shares one request can hold it in scalar registers.

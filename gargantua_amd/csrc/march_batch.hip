@@ -132,8 +132,15 @@ struct Lane<false> {
         return true;
     }
     template <bool UNIFORM, class Tick = NoTick>
-    GARG_DEV void run(const BatchRequest& r, uint32_t budget, Tick = Tick{}) {
-        m.run(r.uniforms, r.scene, true, r.uniforms.compensated != 0u, budget);
+    GARG_DEV void run(const BatchRequest& r, uint32_t budget, Tick tick = Tick{}) {
+        if constexpr (UNIFORM) {
+            // The wave's request: scalar copies of its uniforms and scene.
+            const garg_march_uniforms u = wave_uniform(r.uniforms);
+            const Scene sc = wave_uniform(r.scene);
+            m.run(u, sc, true, u.compensated != 0u, budget, tick);
+        } else {
+            m.run(r.uniforms, r.scene, true, r.uniforms.compensated != 0u, budget);
+        }
     }
     GARG_DEV bool done() const { return m.done; }
     GARG_DEV void finish(const BatchRequest& r, CounterTable& t, uint32_t request, uint32_t pixel) {
@@ -272,22 +279,27 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 
 // Tile-granular persistent march: a wave marches 64 consecutive rays (one
 // 8 x 8 tile) to the end, then takes the next 64. The first round is
-// static (wave w takes rays 64 w onward), so the launch does not open with
-// every wave queueing on one device-scope atomic; later rounds claim from a
-// single head in ray order, which finishes the frame in the order the
-// hardware would dispatch it and keeps the tail short. No ray state
-// outlives a tile, so the kernel needs no more registers than the
-// one-launch-per-frame march (five waves per SIMD): the policy for workloads
-// whose neighbouring rays take similar step counts.
+// static, so the launch does not open with every wave queueing on one
+// device-scope atomic: wave k of workgroup b takes tile k * gridDim.x + b,
+// so consecutive tiles go to consecutive workgroups (and so to different
+// XCDs and CUs, as one-wave workgroups of a plain launch do) rather than
+// the expensive neighbours of a ring or shadow edge all landing on one CU.
+// Later rounds claim from a single head in ray order, which finishes the
+// frame in the order the hardware would dispatch it and keeps the tail
+// short. No ray state outlives a tile, so the kernel needs no more
+// registers than the one-launch-per-frame march: five waves per SIMD for
+// Kerr (96 VGPRs), four for Schwarzschild, whose jump and gather need more
+// than 96 registers without spilling.
 template <bool KERR, int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void march_tiles_kernel(BatchArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KERR ? 5 : 4))) void march_tiles_kernel(
+    BatchArgs a) {
     __shared__ uint32_t s_counts[GARG_BATCH_MAX_REQUESTS * 4];
     __shared__ unsigned long long s_steps[GARG_BATCH_MAX_REQUESTS];
     clear_counters<BLOCK>(s_counts, s_steps, a.request_count);
     CounterTable table{s_counts, s_steps};
 
     const uint32_t lane = __lane_id();
-    uint64_t base = ((uint64_t)blockIdx.x * (BLOCK / 64) + threadIdx.x / 64u) * 64u;
+    uint64_t base = ((uint64_t)(threadIdx.x / 64u) * gridDim.x + blockIdx.x) * 64u;
     while (base < a.total_rays) {
         const uint64_t got = min((uint64_t)64, a.total_rays - base);
         // Request ranges are 64-aligned, so the whole tile is one request.
