@@ -88,7 +88,7 @@ struct garg_particles {
     void* base;
     uint64_t bytes;
     ParticleView view;
-    void* sort_base;  // the sorted exact deposit's keys, order and sort storage
+    void* sort_base;  // the exact deposit's keys, order, staged particles and sort storage
     uint64_t sort_bytes;
     SplatSortBuffers sort;
 };
@@ -103,6 +103,9 @@ struct garg_volume {
     int32_t* momentum;
     uint32_t* mass;
     Voxel* voxels;
+    void* gather_base;  // the exact deposit's cell starts and brick stamps
+    uint64_t gather_bytes;
+    SplatGather gather;
 };
 
 struct garg_tables {
@@ -387,7 +390,7 @@ int garg_particles_create(garg_ctx* ctx, uint32_t count, uint64_t seed, uint32_t
     if (!rc) {
         uint64_t words = align256((uint64_t)count * 4u);
         p->sort.temp_bytes = splat_sort_temp_bytes(count);
-        p->sort_bytes = 4u * words + align256(p->sort.temp_bytes);
+        p->sort_bytes = 20u * words + align256(p->sort.temp_bytes);
         rc = ctx_alloc(ctx, p->sort_bytes, &p->sort_base, "the deposit's sort buffers");
         if (!rc) {
             char* s = static_cast<char*>(p->sort_base);
@@ -395,7 +398,9 @@ int garg_particles_create(garg_ctx* ctx, uint32_t count, uint64_t seed, uint32_t
             p->sort.keys_out = (uint32_t*)(s + words);
             p->sort.order_in = (uint32_t*)(s + 2 * words);
             p->sort.order_out = (uint32_t*)(s + 3 * words);
-            p->sort.temp = s + 4 * words;
+            p->sort.records = (float4*)(s + 4 * words);
+            p->sort.sorted = (float4*)(s + 12 * words);
+            p->sort.temp = s + 20 * words;
         }
     }
     if (rc) {
@@ -528,9 +533,28 @@ int garg_volume_create(garg_ctx* ctx, uint32_t size, float half_extent, float ha
     v->momentum = (int32_t*)(b + voxels * 8u);
     v->mass = (uint32_t*)(b + voxels * 20u);
     v->voxels = (Voxel*)(b + sums);
+    const uint64_t m = (uint64_t)size + 1u;
+    const uint64_t cells = align256(m * m * m * 4u);
+    v->gather_bytes = 2u * cells + align256(splat_gather_bricks(size) * 4u);
+    rc = ctx_alloc(ctx, v->gather_bytes, &v->gather_base, "the deposit's cell tables");
+    if (rc) {
+        ctx_free(ctx, v->base, v->bytes);
+        delete v;
+        return rc;
+    }
+    v->gather.cells = (uint32_t*)v->gather_base;
+    v->gather.ends = (uint32_t*)((char*)v->gather_base + cells);
+    v->gather.bricks = (uint32_t*)((char*)v->gather_base + 2u * cells);
+    v->gather.stamp = 0;
     hipError_t e = launch_fill_zero(v->base, v->bytes, ctx->stream);
+    if (e == hipSuccess) e = launch_fill_zero(v->gather.bricks, v->gather_bytes - 2u * cells, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    HIP_TRY(e, "clearing the volume");
+    if (e != hipSuccess) {
+        ctx_free(ctx, v->base, v->bytes);
+        ctx_free(ctx, v->gather_base, v->gather_bytes);
+        delete v;
+        return fail(GARG_E_DEVICE, "clearing the volume: %s", hipGetErrorString(e));
+    }
     *out = v;
     return GARG_OK;
 }
@@ -540,6 +564,7 @@ int garg_volume_destroy(garg_volume* v) {
     REFUSE_WHILE_CAPTURING(v->ctx, (garg_stream*)nullptr, "garg_volume_destroy");
     (void)hipStreamSynchronize(v->ctx->stream);
     ctx_free(v->ctx, v->base, v->bytes);
+    ctx_free(v->ctx, v->gather_base, v->gather_bytes);
     delete v;
     return GARG_OK;
 }
@@ -570,11 +595,16 @@ int garg_splat_bin(garg_volume* v, garg_particles* p, garg_stream* stream, uint3
     a.seed_hi = (uint32_t)(p->seed >> 32);
     a.frame_counter = frame == GARG_FRAME_AUTO ? p->view.dispatches : nullptr;
     hipStream_t s = stream_of(v->ctx, stream);
-    // The exact deposit sorts and aggregates in LDS (the same integer sums,
-    // far fewer global atomics). Inside a graph capture it deposits
-    // directly: the library sort is not recorded into graphs.
-    if (a.exact && !capturing(s)) HIP_TRY(launch_splat_sorted(a, p->sort, s), "launching the sorted splat");
-    else HIP_TRY(launch_splat_bin(a, s), "launching splatParticles");
+    // The exact deposit sorts and gathers (the same integer sums, no
+    // atomics). Inside a graph capture it deposits directly: the library
+    // sort clears its storage with memset nodes, which graphs must not hold.
+    if (a.exact && !capturing(s)) {
+        // A new stamp per deposit; zero is the cleared bricks' value.
+        if (++v->gather.stamp == 0u) v->gather.stamp = 1u;
+        HIP_TRY(launch_splat_sorted(a, p->sort, v->gather, s), "launching the sorted splat");
+    } else {
+        HIP_TRY(launch_splat_bin(a, s), "launching splatParticles");
+    }
     return GARG_OK;
 }
 

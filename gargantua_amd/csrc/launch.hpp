@@ -124,6 +124,10 @@ hipError_t launch_splat_bin(const SplatArgs& a, hipStream_t stream);
 
 // Buffers of the sorted exact deposit: keys and particle order (in and out
 // of the radix sort, `count` words each) and the sort's temporary storage.
+// The exact deposit's per-particle buffers (in the particle store): cell
+// keys and particle indices before and after the sort, the sort's storage,
+// and each particle's 32-byte record (fractions x y z, energy; temperature
+// ratio, velocity x y z) in particle order and in sorted order.
 struct SplatSortBuffers {
     uint32_t* keys_in;
     uint32_t* keys_out;
@@ -131,9 +135,23 @@ struct SplatSortBuffers {
     uint32_t* order_out;
     void* temp;
     size_t temp_bytes;
+    float4* records;
+    float4* sorted;
+};
+// The exact deposit's per-volume tables: the first sorted index and one past
+// the last of each of the (size + 1)^3 cells (trusted only where the sorted
+// keys confirm them), and one word per voxel brick holding the stamp of the
+// last deposit that touched it.
+struct SplatGather {
+    uint32_t* cells;
+    uint32_t* ends;
+    uint32_t* bricks;
+    uint32_t stamp;
 };
 size_t splat_sort_temp_bytes(uint32_t count);
-hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, hipStream_t stream);
+uint64_t splat_gather_bricks(uint32_t size);
+hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, const SplatGather& g,
+                               hipStream_t stream);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
 hipError_t launch_fill_zero(void* ptr, uint64_t bytes, hipStream_t stream);
 hipError_t launch_atomic_rate(uint32_t* buffer, uint32_t words, uint32_t adds, uint32_t threads, hipStream_t stream);

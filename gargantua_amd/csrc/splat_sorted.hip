@@ -1,16 +1,25 @@
-// The exact (eight-corner) deposit with the atomics aggregated in LDS.
+// The exact (eight-corner) deposit without global atomics.
 //
-// One no-return 32-bit global atomic costs about as much as the whole
-// deposit's arithmetic: the bin is bound by their rate (about 27 G/s on
-// gfx950 to random words, which the splat at 48 atomics per particle nearly
-// reaches). Particles are therefore radix-sorted by the voxel their low CIC
-// corner sits in; a workgroup then deposits a run of sorted particles, which
-// covers a few rows of one z plane, into an LDS box with LDS atomics and
-// flushes each non-zero box word with one global atomic. The sums are
-// integers, so the result is bit-identical to the direct deposit whatever
-// the order. A run whose box does not fit (it crosses a z plane, or spans
-// too many rows) deposits directly, like splat_kernel.
-#include <hipcub/hipcub.hpp>
+// Device-scope atomics on gfx950 are performed past the per-XCD L2 (every
+// one reaches the memory side), about 27 G/s to random words; the direct
+// deposit issues 48 per particle and is bound by them. Here particles are
+// radix-sorted (onesweep) by the CIC cell their low corner sits in, with a
+// 32-byte record of each particle's corner data carried into sorted order.
+// The voxels are then split into bricks, each owned by one workgroup: it
+// reads the sorted runs of the cells whose corners can land in the brick,
+// deposits those corners into an LDS box with LDS atomics, and adds the box
+// to the brick's sums with plain read-modify-writes, since no other
+// workgroup writes them. The per-corner contributions are the same float
+// operations as the direct deposit (this file is built with contraction
+// off) and the sums are integers added modulo 2^32, so the result is
+// bit-identical to the direct deposit's, whatever the order.
+//
+// Cells are indexed by the floor of the voxel-centre grid coordinate plus
+// one on each axis, c in [0, size], in a (size + 1)^3 table (z major). A
+// cell c deposits on each axis to max(c - 1, 0) (weight 1 - fraction) and
+// min(c, size - 1) (weight fraction), so voxel v draws from cells v and
+// v + 1 only.
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "device_common.hpp"
 #include "launch.hpp"
@@ -19,15 +28,11 @@ namespace garg {
 
 namespace {
 
-constexpr uint32_t SORTED_BLOCK = 256;
-constexpr uint32_t BOX_WORDS = 12288;  // 48 KB of LDS
-
-// Sorted particles per thread: a workgroup's run should cover few enough
-// rows of one z plane for its box (rows x size x 6 words) to fit, and a
-// finer grid puts fewer particles in a row.
-__host__ __device__ inline uint32_t sorted_per_thread(uint32_t size) {
-    return size <= 128u ? 8u : size <= 192u ? 4u : size <= 256u ? 2u : 1u;
-}
+// Voxel bricks of the deposit, one workgroup each, and the consecutive
+// sorted particles one thread deposits.
+constexpr uint32_t BRICK_X = 16, BRICK_Y = 8, BRICK_Z = 4;
+constexpr uint32_t BRICK_THREADS = 512;
+constexpr uint32_t CHUNK = 8;
 
 struct Corners {
     int low[3], high[3];
@@ -77,12 +82,13 @@ struct Contribution {
     int32_t mx, my, mz;
 };
 
-GARG_DEV Contribution contribution(float weight, const Corners& c) {
+// splat_kernel's deposit, returning the integers instead of adding them.
+GARG_DEV Contribution contribution(float weight, float energy, float relative, float vx, float vy, float vz) {
     Contribution out;
-    float e = c.energy * weight;
+    float e = energy * weight;
     e = e * GARG_F(ENERGY_FIXED);
     e = e + 0.5f;
-    float et = c.energy * c.relative;
+    float et = energy * relative;
     et = et * weight;
     et = et * GARG_F(ENERGY_T_FIXED);
     et = et + 0.5f;
@@ -91,181 +97,274 @@ GARG_DEV Contribution contribution(float weight, const Corners& c) {
     m = m + 0.5f;
     out.e = (uint32_t)e;
     out.et = (uint32_t)et;
-    out.mx = (int32_t)rintf(c.vx * wm);
-    out.my = (int32_t)rintf(c.vy * wm);
-    out.mz = (int32_t)rintf(c.vz * wm);
+    out.mx = (int32_t)rintf(vx * wm);
+    out.my = (int32_t)rintf(vy * wm);
+    out.mz = (int32_t)rintf(vz * wm);
     out.mass = (uint32_t)m;
     return out;
 }
 
-GARG_DEV float corner_weight(const Corners& c, uint32_t k) {
-    float wx = (k & 1u) ? c.fraction[0] : 1.0f - c.fraction[0];
-    float wy = (k & 2u) ? c.fraction[1] : 1.0f - c.fraction[1];
-    float wz = (k & 4u) ? c.fraction[2] : 1.0f - c.fraction[2];
+// splat_kernel's corner weight: (wx * wy) * wz.
+GARG_DEV float corner_weight(float fx, float fy, float fz, uint32_t k) {
+    float wx = (k & 1u) ? fx : 1.0f - fx;
+    float wy = (k & 2u) ? fy : 1.0f - fy;
+    float wz = (k & 4u) ? fz : 1.0f - fz;
     float w = wx * wy;
     return w * wz;
 }
 
-GARG_DEV void global_add_u32(uint32_t* p, uint32_t v) {
-    if (v) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-GARG_DEV void global_add_i32(int32_t* p, int32_t v) {
-    if (v) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+GARG_DEV uint32_t brick_of(uint32_t x, uint32_t y, uint32_t z, uint32_t nbx, uint32_t nby) {
+    return ((z / BRICK_Z) * nby + y / BRICK_Y) * nbx + x / BRICK_X;
 }
 
 }  // namespace
 
-// Sort key of a particle: the floor of its voxel-centre grid coordinate plus
-// one on each axis, in a (size + 1)^3 index (z major); particles outside the
-// cube get (size + 1)^3 and sort last.
-__global__ __launch_bounds__(256) void splat_key_kernel(SplatArgs a, uint32_t* keys, uint32_t* order) {
+// Sort key of a particle: its cell, z major; particles outside the cube get
+// (size + 1)^3 and sort last. The particle's staged record (fractions x y
+// z, energy, temperature ratio, velocity x y z) is written in particle
+// order here, where the reads are coalesced.
+__global__ __launch_bounds__(256) void splat_key_kernel(SplatArgs a, SplatSortBuffers b) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= a.p.count) return;
     const uint32_t m = a.size + 1u;
     Corners c;
     int f[3];
     uint32_t key = m * m * m;
-    if (corners_of(a, i, c, f)) key = ((uint32_t)(f[2] + 1) * m + (uint32_t)(f[1] + 1)) * m + (uint32_t)(f[0] + 1);
-    keys[i] = key;
-    order[i] = i;
+    if (corners_of(a, i, c, f)) {
+        key = ((uint32_t)(f[2] + 1) * m + (uint32_t)(f[1] + 1)) * m + (uint32_t)(f[0] + 1);
+        b.records[2u * i + 0u] = make_float4(c.fraction[0], c.fraction[1], c.fraction[2], c.energy);
+        b.records[2u * i + 1u] = make_float4(c.relative, c.vx, c.vy, c.vz);
+    }
+    b.keys_in[i] = key;
+    b.order_in[i] = i;
 }
 
-__global__ __launch_bounds__(SORTED_BLOCK) void splat_sorted_kernel(SplatArgs a, const uint32_t* keys,
-                                                                   const uint32_t* order, uint32_t per_thread) {
-    __shared__ uint32_t box[BOX_WORDS];
-    __shared__ uint32_t s_plan[4];  // z0, y0, rows (0: deposit directly), valid count
-    const uint32_t m = a.size + 1u;
-    const uint32_t outside = m * m * m;
-    const uint32_t first = blockIdx.x * SORTED_BLOCK * per_thread;
-    const uint32_t last_index = min(first + SORTED_BLOCK * per_thread, a.p.count) - 1u;
-    const uint32_t n = a.size;
+// Move each particle's record into sorted order; the first and last
+// particles of each cell record the cell's range, and the first stamps the
+// bricks of the voxels the cell deposits into.
+__global__ __launch_bounds__(256) void splat_stage_kernel(SplatArgs a, SplatSortBuffers b, SplatGather g) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t count = a.p.count;
+    if (s >= count) return;
+    const uint32_t n = a.size, m = n + 1u;
+    const uint32_t key = b.keys_out[s];
+    if (key == m * m * m) return;
+    const uint32_t i = b.order_out[s];
+    b.sorted[2u * s + 0u] = b.records[2u * i + 0u];
+    b.sorted[2u * s + 1u] = b.records[2u * i + 1u];
+    if (s + 1u == count || b.keys_out[s + 1u] != key) g.ends[key] = s + 1u;
+    if (s > 0 && b.keys_out[s - 1u] == key) return;
+    g.cells[key] = s;
+    const int cx = (int)(key % m), cy = (int)((key / m) % m), cz = (int)(key / (m * m));
+    const uint32_t nbx = (n + BRICK_X - 1) / BRICK_X, nby = (n + BRICK_Y - 1) / BRICK_Y;
+    // The cell's corners (equal on a clamped axis: the repeated store is
+    // harmless).
+    for (uint32_t k = 0; k < CIC_CORNERS; ++k) {
+        uint32_t x = (uint32_t)((k & 1u) ? min(cx, (int)n - 1) : max(cx - 1, 0));
+        uint32_t y = (uint32_t)((k & 2u) ? min(cy, (int)n - 1) : max(cy - 1, 0));
+        uint32_t z = (uint32_t)((k & 4u) ? min(cz, (int)n - 1) : max(cz - 1, 0));
+        g.bricks[brick_of(x, y, z, nbx, nby)] = g.stamp;
+    }
+}
 
-    if (threadIdx.x == 0) {
-        // The run's low corners lie between its first and last keys; with
-        // one z plane they cover rows y0..y1 and their +1 corners y1 + 1 and
-        // z0 + 1, clamped to the grid.
-        uint32_t k0 = keys[first];
-        uint32_t k1 = keys[last_index];
-        uint32_t rows = 0, z0 = 0, y0 = 0;
-        if (k0 != outside) {
-            if (k1 == outside) {
-                // Only the valid prefix counts: find its last key.
-                uint32_t lo = first, hi = last_index;  // keys[lo] valid, keys[hi] outside
-                while (hi - lo > 1u) {
-                    uint32_t mid = (lo + hi) >> 1;
-                    if (keys[mid] == outside) hi = mid;
-                    else lo = mid;
-                }
-                k1 = keys[lo];
-            }
-            uint32_t fz0 = k0 / (m * m), fy0 = (k0 / m) % m;
-            uint32_t fz1 = k1 / (m * m), fy1 = (k1 / m) % m;
-            if (fz0 == fz1) {
-                // Low corners at max(f, 0) and high corners at min(f + 1, n - 1)
-                // in grid units, where f = key coordinate - 1.
-                int ylo = max((int)fy0 - 1, 0), yhi = min((int)fy1, (int)n - 1);
-                int zlo = max((int)fz0 - 1, 0), zhi = min((int)fz0, (int)n - 1);
-                uint32_t r = (uint32_t)(yhi - ylo + 1) * (uint32_t)(zhi - zlo + 1);
-                if ((uint64_t)r * n * 6u <= BOX_WORDS) {
-                    rows = r;
-                    z0 = (uint32_t)zlo;
-                    y0 = (uint32_t)ylo;
-                    s_plan[3] = (uint32_t)(yhi - ylo + 1);
-                }
-            }
-        }
-        s_plan[0] = z0;
-        s_plan[1] = y0;
-        s_plan[2] = rows;
+// One workgroup per stamped brick of BRICK_X x BRICK_Y x BRICK_Z voxels. The
+// cells whose corners can land in it are BRICK_X + 1 consecutive keys on
+// each of (BRICK_Y + 1) x (BRICK_Z + 1) rows, so their particles are that
+// many contiguous runs of the sorted order. The workgroup deposits the
+// corners that land in the brick into an LDS box and then adds the box to
+// the brick's sums: it is the only writer of those voxels in the launch.
+__global__ __launch_bounds__(BRICK_THREADS) void splat_brick_kernel(SplatArgs a, SplatSortBuffers b, SplatGather g) {
+    constexpr uint32_t VOXELS = BRICK_X * BRICK_Y * BRICK_Z;
+    constexpr uint32_t ROWS = (BRICK_Y + 1) * (BRICK_Z + 1);
+    constexpr uint32_t CELLS = (BRICK_X + 1) * ROWS;
+    __shared__ uint32_t box[6 * VOXELS];
+    __shared__ uint32_t row_lo[ROWS], row_hi[ROWS], prefix[ROWS + 1];
+    if (g.bricks[blockIdx.x] != g.stamp) return;
+    const uint32_t n = a.size, m = n + 1u;
+    const uint32_t nbx = (n + BRICK_X - 1) / BRICK_X, nby = (n + BRICK_Y - 1) / BRICK_Y;
+    const uint32_t x0 = (blockIdx.x % nbx) * BRICK_X, y0 = ((blockIdx.x / nbx) % nby) * BRICK_Y;
+    const uint32_t z0 = (blockIdx.x / (nbx * nby)) * BRICK_Z;
+    const uint32_t count = a.p.count;
+    for (uint32_t w = threadIdx.x; w < 6 * VOXELS; w += BRICK_THREADS) box[w] = 0u;
+    for (uint32_t r = threadIdx.x; r < ROWS; r += BRICK_THREADS) {
+        row_lo[r] = 0xffffffffu;
+        row_hi[r] = 0u;
     }
     __syncthreads();
-    const uint32_t rows = s_plan[2];
-    const uint32_t z0 = s_plan[0], y0 = s_plan[1];
-    const uint32_t ycount = rows ? s_plan[3] : 1u;
-    const uint32_t box_words = rows * n * 6u;
-    for (uint32_t w = threadIdx.x; w < box_words; w += SORTED_BLOCK) box[w] = 0u;
+    // The sorted range of each row: from the first occupied cell's start to
+    // the last one's end. A start or end is trusted only when the sorted
+    // keys confirm it, so the tables need no clearing between deposits.
+    for (uint32_t c = threadIdx.x; c < CELLS; c += BRICK_THREADS) {
+        const uint32_t cx = x0 + c % (BRICK_X + 1), row = c / (BRICK_X + 1);
+        const uint32_t cy = y0 + row % (BRICK_Y + 1), cz = z0 + row / (BRICK_Y + 1);
+        if (cx > n || cy > n || cz > n) continue;
+        const uint32_t cell = (cz * m + cy) * m + cx;
+        const uint32_t s = g.cells[cell];
+        if (s >= count || b.keys_out[s] != cell || (s > 0 && b.keys_out[s - 1u] == cell)) continue;
+        atomicMin(&row_lo[row], s);
+        atomicMax(&row_hi[row], g.ends[cell]);
+    }
     __syncthreads();
-
-    // Each thread deposits `per_thread` consecutive sorted particles,
-    // so the lanes of one LDS atomic are that many particles apart and
-    // rarely share a voxel (neighbours in the sorted order mostly do).
-    for (uint32_t j = 0; j < per_thread; ++j) {
-        const uint32_t s = first + threadIdx.x * per_thread + j;
-        if (s > last_index || keys[s] == outside) continue;
-        const uint32_t i = order[s];
-        Corners c;
-        int f[3];
-        corners_of(a, i, c, f);
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t r = 0; r < ROWS; ++r) {
+            prefix[r] = total;
+            if (row_hi[r] > row_lo[r]) total += row_hi[r] - row_lo[r];
+        }
+        prefix[ROWS] = total;
+    }
+    __syncthreads();
+    const uint32_t total = prefix[ROWS];
+    // Each thread takes CHUNK consecutive particles of the rows (a cell's
+    // particles are consecutive) and sums each cell's eight corners in
+    // registers, adding them to the box once per cell: LDS atomics to one
+    // address from many lanes serialise, and a cell holds several particles.
+    for (uint32_t f0 = threadIdx.x * CHUNK; f0 < total; f0 += BRICK_THREADS * CHUNK) {
+        const uint32_t f1 = min(f0 + CHUNK, total);
+        uint32_t row = 0, hi = ROWS;  // the row r with prefix[r] <= f0 < prefix[r + 1]
+        while (hi - row > 1u) {
+            const uint32_t mid = (row + hi) >> 1;
+            if (prefix[mid] <= f0) row = mid;
+            else hi = mid;
+        }
+        uint32_t s = row_lo[row] + (f0 - prefix[row]);
+        uint32_t key = 0xffffffffu;
+        uint32_t inside = 0, at[CIC_CORNERS];
+        uint32_t acc[CIC_CORNERS][6];
+        // The next particle's key and record are loaded one step ahead, so
+        // their latency overlaps the current particle's arithmetic.
+        uint32_t next_key = b.keys_out[s];
+        float4 next0 = b.sorted[2u * s + 0u], next1 = b.sorted[2u * s + 1u];
+        for (uint32_t f = f0; f < f1; ++f) {
+            const uint32_t cell = next_key;
+            const float4 r0 = next0, r1 = next1;
+            // Step to the particle after: the same row, or the start of the
+            // next non-empty one.
+            ++s;
+            if (f + 1u < f1) {
+                if (f + 1u == prefix[row + 1u]) {
+                    do {
+                        ++row;
+                    } while (prefix[row + 1u] == prefix[row]);
+                    s = row_lo[row];
+                }
+                next_key = b.keys_out[s];
+                next0 = b.sorted[2u * s + 0u];
+                next1 = b.sorted[2u * s + 1u];
+            }
+            if (cell != key) {
+                // Flush the finished cell, then set up this one.
+                if (key != 0xffffffffu) {
+#pragma unroll
+                    for (uint32_t k = 0; k < CIC_CORNERS; ++k) {
+                        if (!((inside >> k) & 1u)) continue;
+#pragma unroll
+                        for (uint32_t q = 0; q < 6; ++q)
+                            if (acc[k][q]) atomicAdd(&box[q * VOXELS + at[k]], acc[k][q]);
+                    }
+                }
+                key = cell;
+                const int cx = (int)(key % m), cy = (int)((key / m) % m), cz = (int)(key / (m * m));
+                inside = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < CIC_CORNERS; ++k) {
+                    const int vx = (k & 1u) ? min(cx, (int)n - 1) : max(cx - 1, 0);
+                    const int vy = (k & 2u) ? min(cy, (int)n - 1) : max(cy - 1, 0);
+                    const int vz = (k & 4u) ? min(cz, (int)n - 1) : max(cz - 1, 0);
+                    const uint32_t lx = (uint32_t)(vx - (int)x0), ly = (uint32_t)(vy - (int)y0);
+                    const uint32_t lz = (uint32_t)(vz - (int)z0);
+                    at[k] = (lz * BRICK_Y + ly) * BRICK_X + lx;
+                    if (lx < BRICK_X && ly < BRICK_Y && lz < BRICK_Z) inside |= 1u << k;
+#pragma unroll
+                    for (uint32_t q = 0; q < 6; ++q) acc[k][q] = 0u;
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < CIC_CORNERS; ++k) {
+                if (!((inside >> k) & 1u)) continue;
+                const float w = corner_weight(r0.x, r0.y, r0.z, k);
+                if (w <= 0.0f) continue;
+                const Contribution v = contribution(w, r0.w, r1.x, r1.y, r1.z, r1.w);
+                acc[k][0] += v.e;
+                acc[k][1] += v.et;
+                acc[k][2] += (uint32_t)v.mx;
+                acc[k][3] += (uint32_t)v.my;
+                acc[k][4] += (uint32_t)v.mz;
+                acc[k][5] += v.mass;
+            }
+        }
 #pragma unroll
         for (uint32_t k = 0; k < CIC_CORNERS; ++k) {
-            int cx = (k & 1u) ? c.high[0] : c.low[0];
-            int cy = (k & 2u) ? c.high[1] : c.low[1];
-            int cz = (k & 4u) ? c.high[2] : c.low[2];
-            float w = corner_weight(c, k);
-            if (w <= 0.0f) continue;
-            Contribution v = contribution(w, c);
-            if (rows) {
-                // Box word of sum q at (x, y, z): ((q * rows + row) * n + x),
-                // row = (z - z0) * ycount + (y - y0).
-                uint32_t row = ((uint32_t)cz - z0) * ycount + ((uint32_t)cy - y0);
-                uint32_t base = row * n + (uint32_t)cx;
-                const uint32_t plane = rows * n;
-                if (v.e) atomicAdd(&box[base], v.e);
-                if (v.et) atomicAdd(&box[plane + base], v.et);
-                if (v.mx) atomicAdd(&box[2u * plane + base], (uint32_t)v.mx);
-                if (v.my) atomicAdd(&box[3u * plane + base], (uint32_t)v.my);
-                if (v.mz) atomicAdd(&box[4u * plane + base], (uint32_t)v.mz);
-                if (v.mass) atomicAdd(&box[5u * plane + base], v.mass);
-            } else {
-                uint32_t index = ((uint32_t)cz * n + (uint32_t)cy) * n + (uint32_t)cx;
-                global_add_u32(&a.emission[2u * index + 0u], v.e);
-                global_add_u32(&a.emission[2u * index + 1u], v.et);
-                global_add_i32(&a.momentum[3u * index + 0u], v.mx);
-                global_add_i32(&a.momentum[3u * index + 1u], v.my);
-                global_add_i32(&a.momentum[3u * index + 2u], v.mz);
-                global_add_u32(&a.mass[index], v.mass);
-            }
+            if (!((inside >> k) & 1u)) continue;
+#pragma unroll
+            for (uint32_t q = 0; q < 6; ++q)
+                if (acc[k][q]) atomicAdd(&box[q * VOXELS + at[k]], acc[k][q]);
         }
     }
-    if (!rows) return;
     __syncthreads();
-    // Flush: one global atomic per non-zero box word.
-    const uint32_t plane = rows * n;
-    for (uint32_t w = threadIdx.x; w < plane; w += SORTED_BLOCK) {
-        uint32_t row = w / n, x = w % n;
-        uint32_t z = z0 + row / ycount, y = y0 + row % ycount;
-        uint32_t index = (z * n + y) * n + x;
-        global_add_u32(&a.emission[2u * index + 0u], box[w]);
-        global_add_u32(&a.emission[2u * index + 1u], box[plane + w]);
-        global_add_i32(&a.momentum[3u * index + 0u], (int32_t)box[2u * plane + w]);
-        global_add_i32(&a.momentum[3u * index + 1u], (int32_t)box[3u * plane + w]);
-        global_add_i32(&a.momentum[3u * index + 2u], (int32_t)box[4u * plane + w]);
-        global_add_u32(&a.mass[index], box[5u * plane + w]);
+    for (uint32_t at = threadIdx.x; at < VOXELS; at += BRICK_THREADS) {
+        const uint32_t x = x0 + at % BRICK_X, y = y0 + (at / BRICK_X) % BRICK_Y, z = z0 + at / (BRICK_X * BRICK_Y);
+        if (x >= n || y >= n || z >= n) continue;
+        const uint32_t e = box[at], et = box[VOXELS + at], mx = box[2 * VOXELS + at], my = box[3 * VOXELS + at];
+        const uint32_t mz = box[4 * VOXELS + at], mass = box[5 * VOXELS + at];
+        if ((e | et | mx | my | mz | mass) == 0u) continue;
+        const uint32_t index = (z * n + y) * n + x;
+        uint32_t* em = a.emission + 2u * index;
+        uint32_t* mo = reinterpret_cast<uint32_t*>(a.momentum) + 3u * index;
+        em[0] += e;
+        em[1] += et;
+        mo[0] += mx;
+        mo[1] += my;
+        mo[2] += mz;
+        a.mass[index] += mass;
     }
 }
 
+namespace {
+
+// Onesweep at every size above one block: rocprim's default picks a merge
+// sort up to 2^20 items, several times slower for these keys.
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
+
+int key_bits(uint32_t size) {
+    const uint64_t m = (uint64_t)size + 1u;
+    int bits = 1;
+    while ((1ull << bits) <= m * m * m) ++bits;
+    return bits;
+}
+
+}  // namespace
+
+// Storage for the largest key (32 bits), so one allocation serves every
+// volume size.
 size_t splat_sort_temp_bytes(uint32_t count) {
     size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, count, 0, 32);
+    (void)rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                (const uint32_t*)nullptr, (uint32_t*)nullptr, count, 0, 32);
     return bytes;
 }
 
-hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, hipStream_t stream) {
+uint64_t splat_gather_bricks(uint32_t size) {
+    return (uint64_t)((size + BRICK_X - 1) / BRICK_X) * ((size + BRICK_Y - 1) / BRICK_Y) *
+           ((size + BRICK_Z - 1) / BRICK_Z);
+}
+
+hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, const SplatGather& g,
+                               hipStream_t stream) {
     const uint32_t count = a.p.count;
-    splat_key_kernel<<<dim3((count + 255u) / 256u), 256, 0, stream>>>(a, b.keys_in, b.order_in);
+    const uint32_t blocks = (count + 255u) / 256u;
+    splat_key_kernel<<<dim3(blocks), 256, 0, stream>>>(a, b);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const uint64_t m = (uint64_t)a.size + 1u;
-    int bits = 1;
-    while ((1ull << bits) <= m * m * m) ++bits;
     size_t temp = b.temp_bytes;
-    e = hipcub::DeviceRadixSort::SortPairs(b.temp, temp, b.keys_in, b.keys_out, b.order_in, b.order_out, count, 0, bits,
-                                           stream);
+    e = rocprim::radix_sort_pairs<SortConfig>(b.temp, temp, b.keys_in, b.keys_out, b.order_in, b.order_out, count, 0,
+                                              key_bits(a.size), stream);
     if (e != hipSuccess) return e;
-    const uint32_t per = sorted_per_thread(a.size);
-    const uint32_t run = SORTED_BLOCK * per;
-    splat_sorted_kernel<<<dim3((count + run - 1u) / run), SORTED_BLOCK, 0, stream>>>(a, b.keys_out, b.order_out, per);
+    splat_stage_kernel<<<dim3(blocks), 256, 0, stream>>>(a, b, g);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    splat_brick_kernel<<<dim3((uint32_t)splat_gather_bricks(a.size)), BRICK_THREADS, 0, stream>>>(a, b, g);
     return hipGetLastError();
 }
 

@@ -100,8 +100,9 @@ def test_seed_matches_oracle_spawn(ctx):
 @pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("size,hz", [(96, 30.0), (128, 3.0), (256, 3.0), (12, 30.0)])
 def test_splat_bit_exact_against_cpu_replay(ctx, exact, size, hz):
-    """The exact deposit sorts particles and aggregates in LDS; a 12^3 grid
-    makes most runs cross z planes, which deposit directly."""
+    """The exact deposit sorts particles by cell and deposits one voxel
+    brick per workgroup; a 12^3 grid leaves partial 16 x 8 x 4 bricks on two
+    axes."""
     p = settled(ctx, 250_000, 3, 100)
     v = api.Volume(ctx, size, 30.0, hz)
     v.splat(p, exact=exact, frame=17)
@@ -156,7 +157,7 @@ def test_resolve_against_float64_restatement(ctx):
 
 def test_sorted_exact_deposit_equals_direct_deposit(ctx):
     """Inside a graph capture the exact deposit takes the direct path (no
-    library sort in graphs); eagerly it sorts and aggregates in LDS. Both
+    library sort in graphs); eagerly it sorts and deposits per brick. Both
     must give the same integer sums."""
     p = settled(ctx, 300_000, 5, 60)
     eager = api.Volume(ctx, 128, 30.0, 3.0)
@@ -173,10 +174,22 @@ def test_sorted_exact_deposit_equals_direct_deposit(ctx):
         s.sync()
         eager.splat(p, exact=True)
         ctx.sync()
+    # A second, smaller and denser particle set into the same volumes: the
+    # cell tables still hold the first set's starts, which must be ignored.
+    q = settled(ctx, 70_000, 6, 20, api.GENERATOR_ANNULUS)
+    ctx.graph_begin(s)
+    try:
+        graphed.splat(q, exact=True, stream=s)
+    finally:
+        g2 = ctx.graph_end(s)
+    g2.launch(s)
+    s.sync()
+    eager.splat(q, exact=True)
+    ctx.sync()
     e1, m1, a1 = eager.read_sums()
     e2, m2, a2 = graphed.read_sums()
     assert a1.sum() > 0
     assert np.array_equal(e1, e2) and np.array_equal(m1, m2) and np.array_equal(a1, a2)
-    for o in (g, s, eager, graphed, p):
+    for o in (g, g2, s, eager, graphed, p, q):
         o.close()
 
