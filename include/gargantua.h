@@ -204,7 +204,10 @@ int garg_volume_clear(garg_volume* volume, garg_stream* stream);
  * frame = GARG_FRAME_AUTO takes the frame from the particle store's dispatch
  * count when the kernel runs, so a captured graph of step + splat + resolve
  * draws a new frame on every replay (the reference passes a per-call
- * counter, SplatPass.swift:18). */
+ * counter, SplatPass.swift:18). The exact deposit uses the particle store's
+ * sort buffers and the volume's cell tables, so two exact deposits that
+ * share either must not run concurrently; inside a graph capture it takes
+ * the direct atomic path, which needs neither. */
 #define GARG_FRAME_AUTO 0xffffffffu
 int garg_splat_bin(garg_volume* volume, garg_particles* particles, garg_stream* stream, uint32_t exact, uint32_t frame);
 /* resolveVolume (splat.metal:172-199); clear_sums = 1 also zeroes the sums. */
