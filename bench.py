@@ -355,7 +355,7 @@ def mixed_leg(ctx, api, vol, args):
 
 def pass_leg(ctx, api, particles, vol):
     """Disk passes at 1M particles into 128^3: one sim dispatch (two
-    substeps), the exact splat bin (sorted, and the direct path it replaces)
+    substeps; also fifty in one call), the exact splat bin (sorted, and the direct path it replaces)
     and the resolve with its fused clear, against their algorithmic bytes."""
     n = 1_000_000
     voxels = 128 ** 3
@@ -372,6 +372,9 @@ def pass_leg(ctx, api, particles, vol):
         return a.elapsed_ms(b) / reps
 
     sim_ms = timed(lambda: particles.step(1), 50)
+    # 50 consecutive dispatches in one call: one launch with the particles
+    # in registers (the settle and any multi-dispatch step take this path).
+    fused_ms = timed(lambda: particles.step(50), 4) / 50
     bin_ms = timed(lambda: vol.splat(particles, exact=True), 10)
     # The direct exact deposit (one global atomic per corner sum): inside a
     # graph capture the exact bin takes that path, so time its replays.
@@ -389,7 +392,8 @@ def pass_leg(ctx, api, particles, vol):
     # x 6 sums per particle) to random words of the sums' footprint.
     ceiling = ctx.atomic_rate(6 * voxels, 48, n)
     return {"sim": {"ms": sim_ms, "particles_per_s": n / (sim_ms * 1e-3), "bytes": sim_bytes,
-                    "GB_per_s": sim_bytes / (sim_ms * 1e-3) / 1e9, "hbm_frac": sim_bytes / (sim_ms * 1e-3) / 8e12},
+                    "GB_per_s": sim_bytes / (sim_ms * 1e-3) / 1e9, "hbm_frac": sim_bytes / (sim_ms * 1e-3) / 8e12,
+                    "fused_50": {"ms_per_dispatch": fused_ms, "particle_dispatches_per_s": n / (fused_ms * 1e-3)}},
             # Exact bin: 8 corners x 6 integer sums per particle. Sorted and
             # deposited per voxel brick, it issues no global atomics, so its
             # corner-sum rate is compared with what direct atomics reach.

@@ -451,7 +451,13 @@ int garg_particles_step(garg_particles* p, garg_stream* stream, uint32_t dispatc
     REQUIRE(p, "particles are required.");
     SimArgs a = sim_args(p, GARG_GENERATOR_DISK, 0.0f);
     hipStream_t s = stream_of(p->ctx, stream);
-    for (uint32_t i = 0; i < dispatches; ++i) HIP_TRY(launch_step(a, s), "launching stepParticles");
+    // Up to 256 dispatches per launch (each launch stays a few milliseconds
+    // at the largest stores).
+    for (uint32_t done = 0; done < dispatches;) {
+        const uint32_t n = std::min(dispatches - done, 256u);
+        HIP_TRY(launch_step(a, n, s), "launching stepParticles");
+        done += n;
+    }
     return GARG_OK;
 }
 

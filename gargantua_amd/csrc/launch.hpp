@@ -91,7 +91,8 @@ struct SimArgs {
 };
 
 hipError_t launch_seed(const SimArgs& a, hipStream_t stream);
-hipError_t launch_step(const SimArgs& a, hipStream_t stream);
+// `dispatches` consecutive sim dispatches in one launch.
+hipError_t launch_step(const SimArgs& a, uint32_t dispatches, hipStream_t stream);
 hipError_t launch_pack_aos(const ParticleView& p, garg_particle* out, hipStream_t stream);
 hipError_t launch_unpack_aos(const ParticleView& p, const garg_particle* in, hipStream_t stream);
 

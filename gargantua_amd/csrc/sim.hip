@@ -94,18 +94,24 @@ __global__ __launch_bounds__(256) void seed_kernel(SimArgs a) {
     a.p.respawns[i] = 0u;
 }
 
-// Two substeps of kick, drag, drift; capture on the swept chord, escape and
-// the plunge speed limit respawn (sim.metal:44-79).
-__global__ __launch_bounds__(256) void step_kernel(SimArgs a) {
+// `dispatches` dispatches of two substeps each of kick, drag, drift;
+// capture on the swept chord, escape and the plunge speed limit respawn
+// (sim.metal:44-79). Particles do not interact and a dispatch carries only
+// position, velocity and the respawn count to the next (the temperature is
+// a function of the final position), so consecutive dispatches run in one
+// launch with the particle in registers: the same float operations in the
+// same order as one launch per dispatch, with one read and one write of
+// the store instead of one per dispatch.
+__global__ __launch_bounds__(256) void step_kernel(SimArgs a, uint32_t dispatches) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    // One thread counts the dispatch; nothing in this launch reads the count.
-    if (i == 0u) a.p.dispatches[0] += 1u;
+    // One thread counts the dispatches; nothing in this launch reads the count.
+    if (i == 0u) a.p.dispatches[0] += dispatches;
     if (i >= a.p.count) return;
     float x = a.p.px[i], y = a.p.py[i], z = a.p.pz[i];
     float vx = a.p.vx[i], vy = a.p.vy[i], vz = a.p.vz[i];
     uint32_t respawns = a.p.respawns[i];
     const float dt = a.dt;
-    for (uint32_t s = 0; s < SIM_SUBSTEPS; ++s) {
+    for (uint32_t s = 0; s < SIM_SUBSTEPS * dispatches; ++s) {
         float r = len3(x, y, z);
         float grad = potential_gradient(r, a.potential);
         float k = -grad / r;
@@ -191,8 +197,8 @@ hipError_t launch_seed(const SimArgs& a, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_step(const SimArgs& a, hipStream_t stream) {
-    step_kernel<<<grid_for(a.p.count), 256, 0, stream>>>(a);
+hipError_t launch_step(const SimArgs& a, uint32_t dispatches, hipStream_t stream) {
+    step_kernel<<<grid_for(a.p.count), 256, 0, stream>>>(a, dispatches);
     return hipGetLastError();
 }
 

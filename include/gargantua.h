@@ -180,7 +180,9 @@ int garg_particles_create(garg_ctx* ctx, uint32_t count, uint64_t seed, uint32_t
 int garg_particles_destroy(garg_particles* particles);
 /* seedParticles (sim.metal:36-42); sigma_z is used by the annulus generator. */
 int garg_particles_seed(garg_particles* particles, garg_stream* stream, uint32_t generator, float sigma_z);
-/* stepParticles (sim.metal:44-79), `dispatches` times; each is SIM_SUBSTEPS substeps. */
+/* stepParticles (sim.metal:44-79), `dispatches` times; each is SIM_SUBSTEPS
+ * substeps. Consecutive dispatches run in one launch (up to 256 per launch)
+ * with each particle held in registers: the same bits as one launch each. */
 int garg_particles_step(garg_particles* particles, garg_stream* stream, uint32_t dispatches);
 int garg_particles_reseed(garg_particles* particles, uint64_t seed);
 /* Synchronous AoS export / import of `count` records (Validate.swift:88-100). */

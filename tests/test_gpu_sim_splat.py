@@ -77,6 +77,29 @@ def test_sim_dispatch_bit_exact_against_cpu_replay(ctx, generator):
     p.close()
 
 
+@pytest.mark.parametrize("generator", [api.GENERATOR_DISK, api.GENERATOR_ANNULUS])
+def test_fused_dispatches_equal_one_launch_each(ctx, generator):
+    """Consecutive dispatches run in one launch with the particle in
+    registers (300 = a launch of 256 and one of 44): the store must equal
+    one launch per dispatch bit for bit, respawns included, and the
+    dispatch count must advance the same."""
+    fused = settled(ctx, 150_000, 9, 0, generator)
+    single = settled(ctx, 150_000, 9, 0, generator)
+    fused.step(300)
+    for _ in range(300):
+        single.step(1)
+    a, b = fused.read(), single.read()
+    assert a.tobytes() == b.tobytes()
+    assert fused.dispatches() == single.dispatches() == 300
+    assert fused.checksum() == single.checksum()
+    if generator == api.GENERATOR_DISK:
+        # Its inner particles plunge, so respawns happen inside the fused
+        # launches (the annulus settles without any in this span).
+        assert int((a["respawns"] > 0).sum()) > 0
+    fused.close()
+    single.close()
+
+
 def _temperature_scale():
     peak_r = 6.0 * 49.0 / 36.0
     return 8000.0 / O.lib.orc_disk_shape(peak_r, 0.01)
