@@ -140,11 +140,21 @@ def main():
         cfg = ctx.batch_config(block=b, waves_per_simd=w, compaction_period=per, refill_threshold=thr)
     request = [{"uniforms": u, "targets": targets, "volume": vol}]
 
-    # Steps per frame are a property of the scene: count them once.
+    # Steps per frame are a property of the scene: count them once, with
+    # the frame's drift (maximum over each ray's steps of the Carter and
+    # null-residual drifts) against the still budget: over the rays that
+    # escape or run out of budget, and the counter's figure that includes
+    # captured rays (whose gauges diverge at the horizon).
     targets.reset_counters()
     totals = ctx.march_batch(request, cfg)
     steps_per_frame = int(totals.steps)
     rays = int(totals.rays)
+    debug = targets.read()[1]
+    kept = debug[..., 2] != api.RAY_CAPTURED
+    drift = {"budget": float(u.drift_budget), "max_escaped_or_exhausted": float(debug[..., 0][kept].max()),
+             "rays_over_budget_escaped_or_exhausted": int((debug[..., 0][kept] > u.drift_budget).sum()),
+             "rays_over_budget_all": int(totals.over_drift_budget), "captured": int(totals.captured),
+             "exhausted": int(totals.exhausted)}
 
     batched_ms, batched_frames = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, request, cfg), args.warmup,
                                              args.steps)
@@ -197,6 +207,7 @@ def main():
                    "rays": rays, "steps_per_frame": steps_per_frame, "compensated": True,
                    "batch": {"block": cfg.block, "waves_per_simd": cfg.waves_per_simd,
                              "compaction_period": cfg.compaction_period, "refill_threshold": cfg.refill_threshold}},
+        "drift": drift,
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "latency_ms": {"p50": float(np.percentile(batched_frames, 50)),
