@@ -591,15 +591,20 @@ GARG_DEV f3 kerr_position_fast(float a2, float r, float st, float ct, float phi)
     return f3{radial * st * cp, radial * st * sp, r * ct};
 }
 
-GARG_DEV f3 kerr_direction_fast(float r, float st, float ct, float phi, float rdot, float thdot, float phdot) {
-    float sp, cp;
-    fast_sincos(phi, &sp, &cp);
+// The locally measured direction from the sines and cosines of theta and phi.
+GARG_DEV f3 kerr_direction_sc(float r, float st, float ct, float sp, float cp, float rdot, float thdot, float phdot) {
     f3 r_hat{st * cp, st * sp, ct};
     f3 theta_hat{ct * cp, ct * sp, -st};
     f3 phi_hat{-sp, cp, 0.0f};
     f3 local = r_hat * rdot + theta_hat * (r * thdot) + phi_hat * (r * st * phdot);
     float len2 = dot(local, local);
     return len2 > 0.0f ? local * __builtin_amdgcn_rsqf(len2) : r_hat;
+}
+
+GARG_DEV f3 kerr_direction_fast(float r, float st, float ct, float phi, float rdot, float thdot, float phdot) {
+    float sp, cp;
+    fast_sincos(phi, &sp, &cp);
+    return kerr_direction_sc(r, st, ct, sp, cp, rdot, thdot, phdot);
 }
 
 // The right-hand sides at a state whose sin and cos of theta are given.
@@ -689,7 +694,7 @@ struct KerrFastMarch {
     KerrEval e;                      // evaluation at the current state
     float carter0, inv_scale, capture;
     Gather g;
-    f3 prev;
+    float prev_rho, prev_z, prev_ph;  // the previous point, cylindrical
     float drift;
     uint32_t steps;
     uint32_t outcome;
@@ -708,7 +713,9 @@ struct KerrFastMarch {
         inv_scale = 1.0f / fmaxf(fabsf(carter0), k.E * k.E);
         capture = k.rplus + GARG_F(KERR_CAPTURE_MARGIN);
         g.reset();
-        prev = kerr_position_fast(k.a2, r, e.st, e.ct, ph);
+        prev_rho = fast_sqrt(r * r + k.a2) * e.st;
+        prev_z = r * e.ct;
+        prev_ph = ph;
         drift = 0.0f;
         steps = 0;
         outcome = GARG_RAY_EXHAUSTED;
@@ -757,9 +764,15 @@ struct KerrFastMarch {
             if (!compensated) cr = cth = cph = cpr = cpth = 0.0f;
             steps += 1;
             e = kerr_eval(k, r, th, pr, pth);
-            f3 p = kerr_position_fast(k.a2, r, e.st, e.ct, ph);
-            f3 dp = p - prev;
-            float path = fast_sqrt(dot(dp, dp));
+            // The step's chord in cylindrical coordinates (rho, phi, z):
+            // |dp|^2 = drho^2 + 2 rho rho' (1 - cos dphi) + dz^2, where dphi
+            // is the exact difference of the two float angles, so the sine
+            // and cosine of phi are needed only where the gas is sampled.
+            const float rho = fast_sqrt(r * r + k.a2) * e.st;
+            const float z = r * e.ct;
+            const float drho = rho - prev_rho, dz = z - prev_z;
+            const float chord = 2.0f * rho * prev_rho * versine(ph - prev_ph);
+            float path = fast_sqrt((drho * drho + chord) + dz * dz);
             g.travelled += path;
             float carter_drift = fabsf(e.carter - carter0) * inv_scale;
             float null_drift = fabsf(e.null_residual) * inv_scale * fast_rcp(fmaxf(r * r, 1.0f));
@@ -769,14 +782,19 @@ struct KerrFastMarch {
                 done = true;
                 break;
             }
-            if (has_volume) {
+            if (has_volume && fabsf(z) < GARG_F(DISK_SLAB_HALF_HEIGHT)) {
+                float sp, cp;
+                fast_sincos(ph, &sp, &cp);
+                const f3 p{rho * cp, rho * sp, z};
                 float f = 1.0f - 2.0f * r * e.inv_sigma;
                 const KerrEval& ee = e;
-                float rr = r, pp = ph;
+                float rr = r;
                 g.sample(l.volume, l.opacity, l.redshift != 0u, p, path, fmaxf(f, 0.0f),
-                         [&]() { return kerr_direction_fast(rr, ee.st, ee.ct, pp, ee.dr, ee.dth, ee.dph); });
+                         [&]() { return kerr_direction_sc(rr, ee.st, ee.ct, sp, cp, ee.dr, ee.dth, ee.dph); });
             }
-            prev = p;
+            prev_rho = rho;
+            prev_z = z;
+            prev_ph = ph;
             if (r >= l.escape_radius) {
                 outcome = GARG_RAY_ESCAPED;
                 done = true;

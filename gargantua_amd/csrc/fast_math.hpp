@@ -88,4 +88,21 @@ GARG_HD void sincos_shift(float s0, float c0, float d, float x, float* s, float*
     }
 }
 
+// 1 - cos d: its Taylor series to d^8 for |d| <= 1/4 (the first omitted
+// term is below 9e-12 of the result), otherwise from fast_sincos, where the
+// subtraction loses nothing that matters. The series keeps full relative
+// accuracy for tiny d, where 1 - cos d in float would round to zero.
+GARG_HD float versine(float d) {
+    if (fabsf(d) <= GARG_SHIFT_LIMIT) {
+        float d2 = d * d;
+        float p = fmaf(d2, -2.48015873015873016e-05f, 1.38888888888888889e-03f);
+        p = fmaf(d2, p, -4.16666666666666667e-02f);
+        p = fmaf(d2, p, 0.5f);
+        return d2 * p;
+    }
+    float s, c;
+    fast_sincos(d, &s, &c);
+    return 1.0f - c;
+}
+
 }  // namespace garg

@@ -98,3 +98,48 @@ def test_sincos_shift_accuracy():
     print(f"shifted sincos abs error {worst:.2e}, fallback {fallback:.2e}")
     assert worst < 4e-7 and fallback < 3e-7
 
+
+
+VERSINE = r"""
+#include <math.h>
+#include <stdio.h>
+#include "fast_math.hpp"
+int main(void) {
+    /* relative error against 2 sin^2(d / 2) in double, which has no
+       cancellation: tiny steps, the series range and the fallback */
+    double worst_series = 0, worst_fallback = 0;
+    for (long i = 1; i <= 2000000; ++i) {
+        float d = (float)(i * 2.5e-7);           /* (0, 0.5] */
+        double h = sin((double)d * 0.5);
+        double want = 2.0 * h * h;
+        double err = fabs((double)garg::versine(d) - want) / want;
+        double err_neg = fabs((double)garg::versine(-d) - want) / want;
+        if (err_neg > err) err = err_neg;
+        if (d <= 0.25f) { if (err > worst_series) worst_series = err; }
+        else if (err > worst_fallback) worst_fallback = err;
+    }
+    for (int k = -15; k <= -3; ++k) {             /* down to 1e-15 rad (d^2 stays a normal float) */
+        float d = powf(10.0f, (float)k);
+        double h = sin((double)d * 0.5);
+        double err = fabs((double)garg::versine(d) - 2.0 * h * h) / (2.0 * h * h);
+        if (err > worst_series) worst_series = err;
+    }
+    printf("%.3e %.3e\n", worst_series, worst_fallback);
+    return 0;
+}
+"""
+
+
+def test_versine_relative_accuracy():
+    """1 - cos d for the Kerr chord: full relative accuracy for tiny steps
+    (where float 1 - cos d is zero) through the series range, and the
+    fallback beyond it, against 2 sin^2(d / 2) in double."""
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "v.cpp"), os.path.join(d, "v")
+        open(src, "w").write(VERSINE)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+                        os.path.join(ROOT, "gargantua_amd", "csrc"), src, "-o", exe, "-lm"], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    series, fallback = float(out[0]), float(out[1])
+    print(f"versine relative error: series {series:.2e}, fallback {fallback:.2e}")
+    assert series < 6e-7 and fallback < 2e-6
