@@ -45,17 +45,34 @@ GARG_DEV RayState ray_derivative(const RayState& s) {
     return RayState{s.rdot, s.phidot, vdd, rdd, pdd};
 }
 
+// The same right-hand sides with one hardware reciprocal of r in place of
+// the four IEEE divisions (the fast form): -vdot^2/r^2 + r phidot^2 is the
+// vdot equation and, times f, the bracket of the rdot equation.
+GARG_DEV RayState ray_derivative_fast(const RayState& s) {
+    float r = s.r;
+    float ir = __builtin_amdgcn_rcpf(r);
+    float ir2 = ir * ir;
+    float vdd = r * s.phidot * s.phidot - s.vdot * s.vdot * ir2;
+    float f = 1.0f - GARG_F(R_SCHWARZSCHILD) * ir;
+    float rdd = 2.0f * s.rdot * s.vdot * ir2 + f * vdd;
+    float pdd = -2.0f * s.rdot * s.phidot * ir;
+    return RayState{s.rdot, s.phidot, vdd, rdd, pdd};
+}
+
 GARG_DEV RayState ray_axpy(const RayState& s, const RayState& k, float h) {
     return RayState{s.r + k.r * h, s.phi + k.phi * h, s.vdot + k.vdot * h, s.rdot + k.rdot * h, s.phidot + k.phidot * h};
 }
 
-// Classical RK4 increment (h / 6)((k1 + 2 k2) + (2 k3 + k4)).
+// Classical RK4 increment (h / 6)((k1 + 2 k2) + (2 k3 + k4)); FAST takes
+// the fast right-hand sides and h times 1/6.
+template <bool FAST = false>
 GARG_DEV RayState ray_increment(const RayState& s, float h) {
-    RayState k1 = ray_derivative(s);
-    RayState k2 = ray_derivative(ray_axpy(s, k1, 0.5f * h));
-    RayState k3 = ray_derivative(ray_axpy(s, k2, 0.5f * h));
-    RayState k4 = ray_derivative(ray_axpy(s, k3, h));
-    float w = h / 6.0f;
+    auto derivative = [](const RayState& x) { return FAST ? ray_derivative_fast(x) : ray_derivative(x); };
+    RayState k1 = derivative(s);
+    RayState k2 = derivative(ray_axpy(s, k1, 0.5f * h));
+    RayState k3 = derivative(ray_axpy(s, k2, 0.5f * h));
+    RayState k4 = derivative(ray_axpy(s, k3, h));
+    float w = FAST ? h * (1.0f / 6.0f) : h / 6.0f;
     return RayState{((k1.r + k2.r * 2.0f) + (k3.r * 2.0f + k4.r)) * w,
                     ((k1.phi + k2.phi * 2.0f) + (k3.phi * 2.0f + k4.phi)) * w,
                     ((k1.vdot + k2.vdot * 2.0f) + (k3.vdot * 2.0f + k4.vdot)) * w,
@@ -129,6 +146,25 @@ GARG_DEV f3 ray_direction(const PlaneRay& ray) {
     f3 local = radial * (ray.s.rdot / sqrtf(fmaxf(schw_f(ray.s.r), 1e-6f))) + tangential * (ray.s.r * ray.s.phidot);
     float len = length(local);
     return len > 0.0f ? local / len : radial;
+}
+
+// The fast form's position and direction: fast_sincos and hardware
+// reciprocal square roots.
+GARG_DEV f3 ray_position_fast(const PlaneRay& ray) {
+    float c, s;
+    fast_sincos(ray.s.phi, &s, &c);
+    return (ray.e1 * c + ray.e2 * s) * ray.s.r;
+}
+
+GARG_DEV f3 ray_direction_fast(const PlaneRay& ray) {
+    float c, s;
+    fast_sincos(ray.s.phi, &s, &c);
+    f3 radial = ray.e1 * c + ray.e2 * s;
+    f3 tangential = ray.e2 * c - ray.e1 * s;
+    float f = fmaxf(1.0f - GARG_F(R_SCHWARZSCHILD) * __builtin_amdgcn_rcpf(ray.s.r), 1e-6f);
+    f3 local = radial * (ray.s.rdot * __builtin_amdgcn_rsqf(f)) + tangential * (ray.s.r * ray.s.phidot);
+    float len2 = dot(local, local);
+    return len2 > 0.0f ? local * __builtin_amdgcn_rsqf(len2) : radial;
 }
 
 // Table coordinate of b: tables sample b = low + (high - low)(1 - (1 - t)^2)
@@ -209,8 +245,12 @@ GARG_DEV f3 sphere_exit_direction(const PlaneRay& ray, const garg_march_uniforms
     return asymptotic_direction(ray, ray.s.phi + sweep);
 }
 
-// The Schwarzschild loop of march.metal:51-117 as a resumable machine.
-struct SchwMarch {
+// The Schwarzschild loop of march.metal:51-117 as a resumable machine. The
+// reference form (FAST false) evaluates Geodesic.h as written; the fast form
+// takes one hardware reciprocal of r per right-hand side, fast_sincos for
+// the position and hardware square roots, on the same steps.
+template <bool FAST>
+struct SchwMarchT {
     PlaneRay ray;
     RayState carry;
     Gather g;
@@ -224,7 +264,7 @@ struct SchwMarch {
         ray = launched;
         carry = RayState{0, 0, 0, 0, 0};
         g.reset();
-        prev = ray_position(ray);
+        prev = position();
         steps = 0;
         outcome = GARG_RAY_EXHAUSTED;
         left_sphere = false;
@@ -246,17 +286,18 @@ struct SchwMarch {
             }
             float h = ray_step_length(prev.z, ray.s.r, gs);
             RayState before = ray.s, carry_before = carry;
-            ray_apply(ray.s, carry, ray_increment(ray.s, h), compensated);
+            ray_apply(ray.s, carry, ray_increment<FAST>(ray.s, h), compensated);
             steps += 1;
             if (u.sphere_radius > 0.0f && ray.s.r >= u.sphere_radius && ray.s.rdot > 0.0f && before.r < u.sphere_radius) {
                 // Land on the sphere: the sweep table is exact only from there.
                 float hh = h * (u.sphere_radius - before.r) / (ray.s.r - before.r);
                 ray.s = before;
                 carry = carry_before;
-                ray_apply(ray.s, carry, ray_increment(before, hh), compensated);
+                ray_apply(ray.s, carry, ray_increment<FAST>(before, hh), compensated);
             }
-            f3 p = ray_position(ray);
-            float path = length(p - prev);
+            f3 p = position();
+            f3 dp = p - prev;
+            float path = FAST ? __builtin_amdgcn_sqrtf(dot(dp, dp)) : length(dp);
             g.travelled += path;
             if (ray.s.r <= gs.capture_radius) {
                 outcome = GARG_RAY_CAPTURED;
@@ -266,7 +307,7 @@ struct SchwMarch {
             if (has_volume) {
                 const PlaneRay& rr = ray;
                 g.sample(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, schw_f(ray.s.r),
-                         [&]() { return ray_direction(rr); });
+                         [&]() { return FAST ? ray_direction_fast(rr) : ray_direction(rr); });
             }
             prev = p;
             if (u.sphere_radius > 0.0f && ray.s.r >= u.sphere_radius && ray.s.rdot > 0.0f) {
@@ -292,8 +333,11 @@ struct SchwMarch {
     }
 
     GARG_DEV f3 sky_direction(const garg_march_uniforms& u, const Scene& sc) const {
-        return left_sphere ? sphere_exit_direction(ray, u, sc) : ray_direction(ray);
+        return left_sphere ? sphere_exit_direction(ray, u, sc) : direction();
     }
+
+    GARG_DEV f3 position() const { return FAST ? ray_position_fast(ray) : ray_position(ray); }
+    GARG_DEV f3 direction() const { return FAST ? ray_direction_fast(ray) : ray_direction(ray); }
 };
 
 // ---------------------------------------------------------------------------

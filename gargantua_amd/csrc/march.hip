@@ -55,7 +55,7 @@ GARG_DEV PixelResult shade_pixel(const garg_march_uniforms& u, const Scene& sc, 
         out.depth = 1.0f;
         return out;
     }
-    SchwMarch m;
+    SchwMarchT<Form == GARG_INTEGRATOR_FAST> m;
     m.begin(p.ray);
     m.run(u, sc, true, Form == GARG_INTEGRATOR_FAST && u.compensated != 0u, cap);
     bool in = m.falls_in();
@@ -130,11 +130,11 @@ __global__ __launch_bounds__(64) void probe_kernel(ProbeArgs a) {
             out.energy = p.ray.energy;
             out.angular_momentum = p.ray.angular_momentum;
         } else {
-            SchwMarch m;
+            SchwMarchT<Form == GARG_INTEGRATOR_FAST> m;
             m.begin(p.ray);
             m.run(u, sc, false, Form == GARG_INTEGRATOR_FAST && u.compensated != 0u, u.geodesic.step_cap);
-            pos = ray_position(m.ray);
-            d = (m.outcome == GARG_RAY_ESCAPED && !m.falls_in()) ? m.sky_direction(u, sc) : ray_direction(m.ray);
+            pos = m.position();
+            d = (m.outcome == GARG_RAY_ESCAPED && !m.falls_in()) ? m.sky_direction(u, sc) : m.direction();
             out.drift = ray_drift(m.ray);
             out.outcome = m.outcome;
             out.steps = m.steps;

@@ -115,7 +115,7 @@ struct Lane<true> {
 
 template <>
 struct Lane<false> {
-    SchwMarch m;
+    SchwMarchT<true> m;
     // Rays that never enter the integration sphere finish at the jump.
     GARG_DEV bool start(const BatchRequest& r, uint32_t px, uint32_t py, CounterTable& t, uint32_t request,
                         uint32_t pixel) {

@@ -47,9 +47,11 @@ enum { GARG_GENERATOR_DISK = 0, GARG_GENERATOR_ANNULUS = 1 };
 
 /* Integrator forms of the march. REFERENCE evaluates the reference's
  * formulas as written (Geodesic.h, Kerr.h); FAST is this library's form
- * (one sincos and three reciprocals per Runge-Kutta stage, drift gauges
- * shared with the next stage), with TwoSum-compensated state accumulation
- * when the uniforms ask for it. */
+ * (Kerr: one sincos per step, angle addition for the inner stages, three
+ * reciprocals per stage, drift gauges shared with the next stage;
+ * Schwarzschild: one reciprocal of r per stage in place of four divisions),
+ * with TwoSum-compensated state accumulation when the uniforms ask for it.
+ * The batched march always runs the FAST form. */
 enum { GARG_INTEGRATOR_REFERENCE = 0, GARG_INTEGRATOR_FAST = 1 };
 
 typedef struct garg_ctx garg_ctx;
