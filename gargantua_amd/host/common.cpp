@@ -35,7 +35,7 @@ const char* kUsage =
     "usage: gargantua [--preset small|default|max] [--particles N] [--volume 96|128|192|256] [--seed S] "
     "[--strategy march|bake] [--fps-cap 30|60] [--no-upscale] [--spin A] [--camera az,el,dist] "
     "[--still out.png --width W --height H [--sequence N]] | bench [--sweep] | validate | serve [--rate R] "
-    "[--seconds T] [--deadline D] [--frames F] [--tick-rays N] [--verify N] [--report PATH] | version | constants";
+    "[--seconds T] [--deadline D] [--frames F] [--tick-rays N] [--policy srpt|fifo] [--verify N] [--report PATH] | version | constants";
 
 namespace {
 
@@ -174,6 +174,10 @@ std::string parse_options(const std::vector<std::string>& args, Options& o) {
             if (!(err = integer(a, o.serve_sequence_frames)).empty()) return err;
         } else if (a == "--verify") {
             if (!(err = integer(a, o.serve_verify)).empty()) return err;
+        } else if (a == "--policy") {
+            if (!(err = value(a, o.serve_policy)).empty()) return err;
+            if (o.serve_policy != "srpt" && o.serve_policy != "fifo")
+                return "unknown policy " + o.serve_policy + "; choose srpt or fifo.";
         } else if (a == "--report") {
             if (!(err = value(a, o.serve_report)).empty()) return err;
         } else {

@@ -51,6 +51,9 @@ struct Options {
     uint64_t serve_seed = 1;
     int serve_sequence_frames = 240;
     int serve_verify = 0;  // re-render every Nth request alone and compare bits
+    // How a tick's ray budget is shared: "srpt" gives it to the active
+    // requests with the least work left first, "fifo" in arrival order.
+    std::string serve_policy = "srpt";
 };
 
 extern const char* kUsage;

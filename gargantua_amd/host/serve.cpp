@@ -234,7 +234,20 @@ void run_serve(const Options& o) {
         // Every active request starts the tick with nothing issued, also those
         // the tick's ray budget does not reach.
         for (int id : active) all[id].tick_issued = 0;
-        for (int id : active) {
+        // The order the budget is handed out in: shortest remaining work
+        // first (rays left over all frames, a still-budget ray counted as two
+        // since it takes about twice the steps; ties in arrival order), or
+        // arrival order. Ray results do not depend on it.
+        std::vector<int> order(active);
+        if (o.serve_policy == "srpt") {
+            auto work = [&](int id) {
+                const Request& r = all[id];
+                const uint64_t left = (uint64_t)(r.frames - r.frame - 1) * r.rays + (r.rays - r.issued);
+                return r.budget > STEP_CAP_INTERACTIVE ? 2 * left : left;
+            };
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) < work(y); });
+        }
+        for (int id : order) {
             Request& r = all[id];
             if (r.bake) continue;
             if (budget == 0) break;
@@ -372,13 +385,14 @@ void run_serve(const Options& o) {
              "\"ticks\": %llu, "
              "\"latency_p50_s\": %.4f, \"latency_p99_s\": %.4f, \"latency_max_s\": %.4f, \"steps\": %llu, "
              "\"steps_per_s\": %.4e, \"gpu_busy_fraction\": %.3f, \"exhausted_rays\": %llu, \"truncated_rays\": %llu, "
-             "\"verified\": %llu, \"mismatched\": %llu, \"rate\": %.3f, \"arrival_seconds\": %.3f, \"frames\": %d}",
+             "\"verified\": %llu, \"mismatched\": %llu, \"rate\": %.3f, \"arrival_seconds\": %.3f, \"frames\": %d, "
+             "\"policy\": \"%s\"}",
              all.size(), completed, unfinished, (unsigned long long)refused, wall, (unsigned long long)ticks, pct(0.5),
              pct(0.99),
              latency.empty() ? 0.0 : latency.back(), (unsigned long long)total_steps, total_steps / wall,
              gpu_busy / wall, (unsigned long long)exhausted, (unsigned long long)truncated,
              (unsigned long long)verified, (unsigned long long)mismatched, o.serve_rate, o.serve_seconds,
-             o.serve_sequence_frames);
+             o.serve_sequence_frames, o.serve_policy.c_str());
     line(buf);
     if (!o.serve_report.empty()) {
         FILE* f = fopen(o.serve_report.c_str(), "w");

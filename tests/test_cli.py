@@ -29,6 +29,7 @@ def run(*args, timeout=600):
     (["--seed", "-3"], 64, "seed must be an unsigned 64 bit integer"),
     (["--particles"], 64, "--particles needs a value."),
     (["--bogus"], 64, "unknown argument --bogus."),
+    (["serve", "--policy", "lifo"], 64, "unknown policy lifo; choose srpt or fifo."),
     ([], 2, "macOS only"),
 ])
 def test_option_grammar_and_exit_codes(args, code, sentence):
@@ -81,16 +82,17 @@ def test_validate_passes_every_judged_golden():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tick_rays", [4 << 20, 50_000])
-def test_serve_results_equal_alone_and_in_the_mix(tmp_path, tick_rays):
+@pytest.mark.parametrize("tick_rays,policy", [(4 << 20, "srpt"), (50_000, "srpt"), (50_000, "fifo")])
+def test_serve_results_equal_alone_and_in_the_mix(tmp_path, tick_rays, policy):
     """Every request re-rendered alone equals its render in the mix; a small
     per-tick ray budget splits frames over many ticks and leaves requests
-    waiting for budget."""
+    waiting for budget, which each policy hands out in its own order."""
     report = tmp_path / "serve.json"
     r = run("serve", "--preset", "small", "--rate", "6", "--seconds", "3", "--frames", "3", "--verify", "1",
-            "--tick-rays", str(tick_rays), "--report", str(report))
+            "--tick-rays", str(tick_rays), "--policy", policy, "--report", str(report))
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     rep = json.loads(report.read_text())
     assert rep["completed"] > 0 and rep["verified"] > 0 and rep["mismatched"] == 0
     assert rep["latency_p99_s"] >= rep["latency_p50_s"] > 0
+    assert rep["policy"] == policy
