@@ -295,9 +295,33 @@ def schwarzschild_legs(ctx, api, default_volume, cfg, cpu: bool):
         out[name] = leg
         t.close()
     out["small_640x360_256_x8"] = small_batch_leg(ctx, api, small_v, tables, cfg)
+    out["bake_walk_960x540"] = bake_leg(ctx, api, default_volume, tables)
     tables.close()
     small_v.close()
     small_p.close()
+    return out
+
+
+def bake_leg(ctx, api, vol, tables):
+    """The bake strategy at 960x540 on the default volume: one still-policy
+    Schwarzschild march storing up to 96 samples per ray at half-voxel
+    spacing (bakeRays), then the walk that composites the stored samples
+    against the volume (walkRays), each timed over five calls."""
+    w, h = 960, 540
+    cam = api.orbit_camera(azimuth=0.0, elevation=math.radians(10.0), distance=50.0, fov_y_deg=60.0)
+    u = api.uniforms(cam, w, h, api.settings("still"), star_seed=1, volume_half_extent=30.0,
+                     bake_spacing=60.0 / 128 * 0.5)
+    tables.apply(u)
+    bake = api.Bake(ctx, w, h, 96)
+    t = api.Targets(ctx, w, h)
+    bake_ms, _ = time_frames(ctx, api, lambda: bake.rows(u, tables), 1, 5)
+    walk_ms, _ = time_frames(ctx, api, lambda: bake.walk(u, vol, t), 1, 5)
+    headers, _ = bake.read()
+    out = {"bake_ms": bake_ms / 5, "walk_ms": walk_ms / 5, "rays": w * h, "overflow_rays": bake.overflow(),
+           "mean_samples_per_ray": float(headers[:, 0].mean()),
+           "bake_bytes": w * h * (8 * 96 + 16)}
+    bake.close()
+    t.close()
     return out
 
 
