@@ -143,6 +143,58 @@ def test_splat_bit_exact_against_cpu_replay(ctx, exact, size, hz):
     p.close()
 
 
+def _replay(rec, size, hz, exact, frame, seed):
+    n = size ** 3
+    re = np.zeros(2 * n, np.uint32)
+    rm = np.zeros(3 * n, np.int32)
+    ra = np.zeros(n, np.uint32)
+    O.lib.orc_splat_replay(rec.ctypes.data_as(C.c_void_p), len(rec), size, 30.0, hz, 8000.0, 0.0005,
+                           1 if exact else 0, frame, seed, re.ctypes.data_as(C.POINTER(C.c_uint32)),
+                           rm.ctypes.data_as(C.POINTER(C.c_int32)), ra.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return re, rm, ra
+
+
+@pytest.mark.parametrize("count,case", [(1, "inside"), (7, "inside"), (1000, "inside"), (1025, "inside"),
+                                        (33_333, "inside"), (5000, "outside"), (5000, "edges")])
+def test_exact_deposit_edge_cases_against_cpu_replay(ctx, count, case):
+    """Exact deposits of one particle, counts that take the sort's
+    single-block path (up to 1024) and counts that are not a multiple of a
+    workgroup, every particle outside the cube (nothing deposited), and
+    particles on the cube's faces and corners (clamped corners), each into
+    a volume that already holds an earlier deposit: the sums must equal the
+    CPU replay of both deposits."""
+    size, hz = 64, 30.0
+    rng = np.random.default_rng(count)
+    p = settled(ctx, count, 13, 0)
+    rec = p.read()
+    if case == "outside":
+        rec["position"] = rng.uniform(31.0, 40.0, (count, 3)).astype(np.float32) * rng.choice([-1, 1], (count, 3))
+    elif case == "edges":
+        # On and next to the faces: grid coordinates -0.5 .. 0 and n - 1 .. n - 0.5 clamp a corner.
+        cell = 60.0 / size
+        pick = rng.choice([-30.0, -30.0 + 0.25 * cell, 30.0 - 0.25 * cell, 30.0 - 1e-4, 0.0], (count, 3))
+        rec["position"] = pick.astype(np.float32)
+    else:
+        rec["position"] = rng.uniform(-29.0, 29.0, (count, 3)).astype(np.float32)
+    rec["velocity"] = rng.normal(0.0, 0.4, (count, 3)).astype(np.float32)
+    rec["temperature"] = rng.uniform(2000.0, 9000.0, count).astype(np.float32)
+    p.write(rec)
+    first = settled(ctx, 20_000, 5, 10)
+    v = api.Volume(ctx, size, 30.0, hz)
+    v.splat(first, exact=True)
+    v.splat(p, exact=True)
+    e, m, a = v.read_sums()
+    e0, m0, a0 = _replay(first.read(), size, hz, True, 0, 5)
+    e1, m1, a1 = _replay(rec, size, hz, True, 0, 13)
+    assert np.array_equal(e, e0 + e1) and np.array_equal(m, m0 + m1) and np.array_equal(a, a0 + a1)
+    if case == "outside":
+        assert np.array_equal(a1, np.zeros_like(a1))
+    else:
+        assert a1.sum() > 0
+    for o in (v, p, first):
+        o.close()
+
+
 def test_resolve_against_float64_restatement(ctx):
     p = settled(ctx, 250_000, 3, 100)
     v = api.Volume(ctx, 96)
