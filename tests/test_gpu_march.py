@@ -157,6 +157,36 @@ def test_each_request_alone_equals_the_mix(ctx, block, period, threshold):
     assert reqs[2]["targets"].counters().rays == 130 * 70
 
 
+@pytest.mark.parametrize("threshold", [0, 24])
+def test_tiny_and_ragged_images_in_one_batch(ctx, threshold):
+    """Images smaller than one 8 x 8 tile and with ragged edges (1 x 1,
+    5 x 3, 9 x 9, 17 x 1), Kerr and Schwarzschild, batched together and with
+    a one-ray slice: each equals its own plain render bit for bit, and every
+    pixel is counted once."""
+    vol = disk_volume(ctx)
+    tables = api.Tables(ctx)
+    reqs = [kerr_request(ctx, 1, 1, 0.9, 60.0, 50.0, 4096, vol),
+            schw_request(ctx, 5, 3, vol, tables),
+            kerr_request(ctx, 9, 9, 0.6, 30.0, 40.0, 256, vol),
+            kerr_request(ctx, 17, 1, 0.99, 85.0, 60.0, 4096, None)]
+    alone = [render_plain(ctx, r) for r in reqs]
+    for r in reqs:
+        r["targets"].reset_counters()
+    rays = api.request_rays(9, 9)
+    mix = [reqs[0], reqs[1], dict(reqs[2], ray_begin=0, ray_count=1), dict(reqs[2], ray_begin=1, ray_count=0),
+           reqs[3]]
+    ctx.march_batch(mix, ctx.batch_config(compaction_period=8, refill_threshold=threshold))
+    for r, (a, c) in zip(reqs, alone):
+        assert same_bits(a, r["targets"].read())
+        got = r["targets"].counters()
+        assert (got.rays, got.steps) == (c.rays, c.steps)
+        assert got.rays == r["targets"].width * r["targets"].height
+    assert rays >= 81
+    for r in reqs:
+        r["targets"].close()
+    tables.close()
+
+
 def _image_vs_oracle(ctx, req, field, integrator, tables=None, samples=400, seed=3, step_slack=2):
     u = req["uniforms"]
     t = api.Targets(ctx, u.resolution[0], u.resolution[1])
