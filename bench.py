@@ -226,6 +226,8 @@ def main():
     if not args.no_passes:
         line["passes"] = pass_leg(ctx, api, p, vol)
         line["fp64"] = fp64_leg(ctx)
+    if not args.no_mixed:
+        line["kerr_640x360_x8"] = kerr_views_leg(ctx, api, vol, cfg)
     if not args.no_schwarzschild:
         line["schwarzschild"] = schwarzschild_legs(ctx, api, vol, cfg, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
@@ -299,6 +301,40 @@ def schwarzschild_legs(ctx, api, default_volume, cfg, cpu: bool):
     tables.close()
     small_v.close()
     small_p.close()
+    return out
+
+
+def kerr_views_leg(ctx, api, vol, cfg, views=8):
+    """Eight 640x360 views of the metric's scene (a = 0.9, 4096-step still
+    budget, r_obs = 50, inclination 60 deg, the 1M / 128^3 volume) from
+    eight azimuths: one batched launch against eight plain launches in each
+    integrator form. A 640x360 frame (3,600 tiles) holds less than the GPU's
+    5,120 wave slots, so one frame alone leaves most of the GPU idle while
+    its slowest tiles finish."""
+    reqs = []
+    for k in range(views):
+        cam = api.orbit_camera(azimuth=2.0 * math.pi * k / views, elevation=math.radians(30.0), distance=50.0,
+                               fov_y_deg=60.0)
+        u = api.uniforms(cam, 640, 360, api.settings("still", step_cap=4096, escape_radius=60.0), spin=0.9,
+                         compensated=True, star_seed=1, volume_half_extent=30.0)
+        reqs.append({"uniforms": u, "targets": api.Targets(ctx, 640, 360), "volume": vol})
+    for r in reqs:
+        r["targets"].reset_counters()
+    totals = ctx.march_batch(reqs, cfg)
+    steps = int(totals.steps)
+    batched_ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, reqs, cfg), 1, 10)
+    out = {"requests": views, "rays": int(totals.rays), "steps": steps, "batched_ms": batched_ms / 10,
+           "batched_steps_per_s": steps / (batched_ms / 10 * 1e-3)}
+    for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
+        def run_all():
+            for r in reqs:
+                ctx.march_image(r["uniforms"], r["targets"], vol, None, integrator)
+        ms, _ = time_frames(ctx, api, run_all, 1, 5)
+        out[form + "_ms"] = ms / 5
+    out["speedup_vs_plain_reference"] = out["plain_reference_ms"] / out["batched_ms"]
+    out["speedup_vs_plain_fast"] = out["plain_fast_ms"] / out["batched_ms"]
+    for r in reqs:
+        r["targets"].close()
     return out
 
 
