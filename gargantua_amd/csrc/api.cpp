@@ -893,6 +893,10 @@ int garg_batch_config_default(garg_ctx* ctx, garg_batch_config* out) {
     return GARG_OK;
 }
 
+// Bands a request is split into in the batch queue (at most, and only while
+// the table has room for every band).
+static const uint32_t BATCH_BANDS = 4;
+
 int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_request* requests, uint32_t count,
                      const garg_batch_config* config, garg_march_counters* total) {
     REQUIRE(ctx && requests && config, "ctx, requests and config are required.");
@@ -927,8 +931,9 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
         HIP_TRY(hipEventSynchronize(ctx->batch_staging_free[part]), "waiting for the staging buffer");
         BatchRequest* staging = ctx->batch_staging + (size_t)part * GARG_BATCH_MAX_REQUESTS;
         BatchRequest* table = ctx->batch_requests + (size_t)part * GARG_BATCH_MAX_REQUESTS;
-        uint32_t n = 0;
-        uint64_t queue = 0;
+        // This part's requests and their ray ranges.
+        uint32_t members[GARG_BATCH_MAX_REQUESTS], member_rays[GARG_BATCH_MAX_REQUESTS];
+        uint32_t m = 0;
         for (uint32_t i = 0; i < count; ++i) {
             const garg_march_request& r = requests[i];
             if ((r.uniforms.spin > 0.0f) != kerr) continue;
@@ -936,20 +941,45 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
             uint32_t rays = all - r.ray_begin;
             if (r.ray_count) rays = std::min(rays, r.ray_count);
             if (rays == 0) continue;
-            BatchRequest& b = staging[n++];
-            b.uniforms = device_uniforms(&r.uniforms, r.volume);
-            b.scene = make_scene(ctx, r.volume, kerr ? nullptr : r.tables);
-            b.color = r.targets->color;
-            b.debug = r.targets->debug;
-            b.depth = r.targets->depth;
-            b.counters = r.targets->counters;
-            b.tiles_x = (r.uniforms.resolution[0] + 7u) / 8u;
-            b.ray_begin = r.ray_begin;
-            b.queue_begin = queue;
-            b.ray_count = rays;
-            b.pad = 0;
-            // 64-aligned ranges: a claim of 64 rays never spans two requests.
-            queue += ((uint64_t)rays + 63u) & ~(uint64_t)63u;
+            members[m] = i;
+            member_rays[m++] = rays;
+        }
+        // Each request is queued as up to BATCH_BANDS bands of whole tiles
+        // (horizontal bands of its image), the middle bands first and the
+        // requests' bands interleaved: the expensive tiles near a shadow or
+        // photon ring start early instead of at the end of the queue, so the
+        // launch does not end on one request's slowest tiles. The queue
+        // order changes no ray's result.
+        const uint32_t bands = std::max(1u, std::min(BATCH_BANDS, m ? GARG_BATCH_MAX_REQUESTS / m : 1u));
+        uint32_t band_order[BATCH_BANDS];
+        for (uint32_t k = 0; k < bands; ++k) band_order[k] = k;
+        std::stable_sort(band_order, band_order + bands, [&](uint32_t x, uint32_t y) {
+            return std::fabs((double)x - 0.5 * (bands - 1)) < std::fabs((double)y - 0.5 * (bands - 1));
+        });
+        uint32_t n = 0;
+        uint64_t queue = 0;
+        for (uint32_t rank = 0; rank < bands; ++rank) {
+            for (uint32_t j = 0; j < m; ++j) {
+                const garg_march_request& r = requests[members[j]];
+                const uint32_t rays = member_rays[j];
+                const uint32_t step = ((rays + bands - 1) / bands + 63u) & ~63u;
+                const uint32_t first = band_order[rank] * step;
+                if (first >= rays) continue;
+                BatchRequest& b = staging[n++];
+                b.uniforms = device_uniforms(&r.uniforms, r.volume);
+                b.scene = make_scene(ctx, r.volume, kerr ? nullptr : r.tables);
+                b.color = r.targets->color;
+                b.debug = r.targets->debug;
+                b.depth = r.targets->depth;
+                b.counters = r.targets->counters;
+                b.tiles_x = (r.uniforms.resolution[0] + 7u) / 8u;
+                b.ray_begin = r.ray_begin + first;
+                b.queue_begin = queue;
+                b.ray_count = std::min(step, rays - first);
+                b.pad = 0;
+                // 64-aligned ranges: a claim of 64 rays never spans two entries.
+                queue += ((uint64_t)b.ray_count + 63u) & ~(uint64_t)63u;
+            }
         }
         if (n == 0) continue;
         HIP_TRY(hipMemcpyAsync(table, staging, sizeof(BatchRequest) * n, hipMemcpyHostToDevice, s), "uploading the batch");
