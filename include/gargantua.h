@@ -286,9 +286,11 @@ int garg_batch_config_default(garg_ctx* ctx, garg_batch_config* out);
 /* Persistent launches marching every requested ray of every request with
  * work stealing and either live-ray refill or whole-tile claims (see
  * garg_batch_config): one launch for the spinning requests
- * and one for the Schwarzschild ones, at most 256 requests. Each request's
- * targets counters accumulate; `total` (optional) receives the sums over
- * the call, read back synchronously. */
+ * and one for the Schwarzschild ones, at most 256 requests. Each request is
+ * queued as up to four bands of whole tiles, middle bands first and
+ * interleaved across requests; no result depends on the queue order. Each
+ * request's targets counters accumulate; `total` (optional) receives the
+ * sums over the call, read back synchronously. */
 int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_request* requests, uint32_t count,
                      const garg_batch_config* config, garg_march_counters* total);
 
