@@ -89,6 +89,22 @@ struct TileAging {
     }
 };
 
+// Sets the wave's issue priority from the age of its oldest live ray (the
+// lane refill's counterpart of TileAging): 0 under TILE_AGE_STEP, then one
+// level at that age, two at twice and three at four times. `started` is the
+// low word of the 100 MHz real-time counter when the lane's ray started.
+GARG_DEV void age_priority(bool live, uint32_t started) {
+    const uint32_t now = (uint32_t)__builtin_amdgcn_s_memrealtime();
+    uint32_t age = live ? now - started : 0u;
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) age = max(age, (uint32_t)__shfl_xor((int)age, offset));
+    const uint32_t oldest = __builtin_amdgcn_readfirstlane(age);
+    if (oldest > 4u * (uint32_t)TILE_AGE_STEP) __builtin_amdgcn_s_setprio(3);
+    else if (oldest > 2u * (uint32_t)TILE_AGE_STEP) __builtin_amdgcn_s_setprio(2);
+    else if (oldest > (uint32_t)TILE_AGE_STEP) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 template <bool KERR>
 struct Lane;
 
@@ -257,6 +273,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     Lane<KERR> ray;
     bool live = false;
     LaneRay id{0u, 0u};
+    uint32_t started = 0;
 
     for (;;) {
         uint64_t dead = __ballot(!live);
@@ -266,12 +283,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             uint64_t base = 0;
             uint64_t got = claim_rays(a, home, queue, drained, (uint64_t)__popcll(dead), base);
             uint64_t rank = (uint64_t)__popcll(dead & below);
-            if (!live && rank < got) live = start_ray(a, ray, id, table, base + rank);
+            if (!live && rank < got) {
+                live = start_ray(a, ray, id, table, base + rank);
+                started = (uint32_t)__builtin_amdgcn_s_memrealtime();
+            }
         }
         if (__ballot(live) == 0ull) {
             if (drained) break;
             continue;
         }
+        // Issue arbitration favours the same hardware wave slots for as long
+        // as the waves live; the oldest rays go first, as in the tile policy.
+        age_priority(live, started);
         live = march_lanes(a, ray, id, table, live, a.compaction_period);
     }
     flush_counters<BLOCK>(a, s_counts, s_steps);
