@@ -70,6 +70,10 @@ class KerrResult(C.Structure):
     _fields_ = [("state", KerrState), ("outcome", C.c_int), ("steps", i64), ("drift", d)]
 
 
+class KsState(C.Structure):
+    _fields_ = [("x", d), ("y", d), ("z", d), ("px", d), ("py", d), ("pz", d), ("energy", d), ("phase", d)]
+
+
 class Field(C.Structure):
     _fields_ = [("size", C.c_int), ("half_extent", d), ("half_extent_z", d), ("emission", dp),
                 ("velocity", dp)]
@@ -169,6 +173,16 @@ _sig("orc_kerr_bisect_critical_xi", d, d, d, d, P(KerrSettings), C.c_int)
 _sig("orc_kerr_critical_xi", d, d, d)
 _sig("orc_kerr_critical_eta", d, d, d)
 _sig("orc_kerr_photon_orbits", None, d, dp)
+_sig("orc_ks_metric", None, d, d, d, d, dp)
+_sig("orc_ks_radius", d, d, d, d, d)
+_sig("orc_ks_from_bl", None, d, P(KerrState), P(KsState))
+_sig("orc_ks_to_bl", None, d, P(KsState), P(KerrState))
+_sig("orc_ks_null_residual", d, d, P(KsState))
+_sig("orc_ks_step", None, d, P(KsState), d)
+_sig("orc_ks_step_length", d, d, P(KsState), P(KerrSettings))
+_sig("orc_ks_position", None, d, P(KsState), dp)
+_sig("orc_ks_drift", d, d, P(KsState), d, d, d)
+_sig("orc_ks_integrate", None, d, P(KsState), P(KerrSettings), P(KerrResult))
 _sig("orc_field_sample", None, P(Field), dp, dp, dp)
 _sig("orc_luminance", d, dp)
 _sig("orc_side_ratio", d, dp, C.c_int, C.c_int)

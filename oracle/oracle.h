@@ -155,6 +155,24 @@ void orc_kerr_integrate(double spin, const orc_kerr_state* launch, const orc_ker
 double orc_kerr_radius_of(double spin, const double point[3]);
 double orc_kerr_bisect_critical_xi(double spin, double radius, double side, const orc_kerr_settings* settings, int iterations);
 double orc_kerr_critical_xi(double spin, double rc);
+
+/* ---- Kerr in Cartesian Kerr-Schild coordinates (kerr_schild.cpp) --------- */
+typedef struct {
+    double x, y, z, px, py, pz; /* position and covariant spatial momentum */
+    double energy;              /* -p_t */
+    double phase;               /* int a/Delta dr at the launch radius */
+} orc_ks_state;
+
+void orc_ks_metric(double spin, double x, double y, double z, double g[16]);
+double orc_ks_radius(double spin, double x, double y, double z);
+void orc_ks_from_bl(double spin, const orc_kerr_state* s, orc_ks_state* out);
+void orc_ks_to_bl(double spin, const orc_ks_state* s, orc_kerr_state* out);
+double orc_ks_null_residual(double spin, const orc_ks_state* s);
+void orc_ks_step(double spin, orc_ks_state* s, double h);
+double orc_ks_step_length(double spin, const orc_ks_state* s, const orc_kerr_settings* settings);
+void orc_ks_position(double spin, const orc_ks_state* s, double out[3]);
+double orc_ks_drift(double spin, const orc_ks_state* s, double carter0, double l0, double scale);
+void orc_ks_integrate(double spin, const orc_ks_state* launch, const orc_kerr_settings* settings, orc_kerr_result* out);
 double orc_kerr_critical_eta(double spin, double rc);
 void orc_kerr_photon_orbits(double spin, double out[2]);
 

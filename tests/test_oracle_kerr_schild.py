@@ -1,0 +1,150 @@
+"""The FP64 Kerr-Schild restatement (oracle/kerr_schild.cpp) pinned by what
+the mathematics fixes: its metric pulled back to Boyer-Lindquist is Kerr
+(KerrGeodesic.swift's metric, closed form), the launch and the conversion
+back are inverse, spin zero is Schwarzschild, rays off the axis land where
+the Boyer-Lindquist oracle lands them, and rays over the pole, where
+Boyer-Lindquist theta is singular, converge at the reference's steps."""
+import ctypes as C
+import math
+
+import numpy as np
+
+import oracle as O
+
+A = 0.9
+
+
+def bl_metric(a, r, th):
+    s = r * r + a * a * math.cos(th) ** 2
+    d = r * r - 2 * r + a * a
+    s2 = math.sin(th) ** 2
+    g = np.zeros((4, 4))
+    g[0, 0] = -(1 - 2 * r / s)
+    g[0, 3] = g[3, 0] = -2 * a * r * s2 / s
+    g[1, 1] = s / d
+    g[2, 2] = s
+    g[3, 3] = (r * r + a * a + 2 * a * a * r * s2 / s) * s2
+    return g
+
+
+def bl_to_ks(a, t, r, th, ph):
+    """t~ = t + int 2r/Delta dr, phi~ = phi + int a/Delta dr,
+    x + iy = (r + i a) sin(theta) e^{i phi~}, z = r cos(theta)."""
+    rp, rm = 1 + math.sqrt(1 - a * a), 1 - math.sqrt(1 - a * a)
+    big_t = 2 / (rp - rm) * (rp * math.log(abs(r - rp)) - rm * math.log(abs(r - rm)))
+    big_p = a / (rp - rm) * math.log(abs((r - rp) / (r - rm)))
+    c = (r + 1j * a) * math.sin(th) * complex(math.cos(ph + big_p), math.sin(ph + big_p))
+    return np.array([t + big_t, c.real, c.imag, r * math.cos(th)])
+
+
+def ks_metric(a, x, y, z):
+    g = (C.c_double * 16)()
+    O.lib.orc_ks_metric(a, x, y, z, g)
+    return np.array(g[:]).reshape(4, 4)
+
+
+def test_ks_metric_pulls_back_to_boyer_lindquist_kerr():
+    worst = 0.0
+    for a in (0.0, 0.5, 0.9, 0.99):
+        for r, th, ph in ((3.0, 0.4, 2.0), (5.0, 1.0, 0.3), (12.0, 2.2, -1.0), (40.0, 1.5, 3.0)):
+            x0 = np.array([0.0, r, th, ph])
+            jac = np.zeros((4, 4))
+            for k in range(4):
+                e = np.zeros(4)
+                e[k] = 1e-6
+                jac[:, k] = (bl_to_ks(a, *(x0 + e)) - bl_to_ks(a, *(x0 - e))) / 2e-6
+            p = bl_to_ks(a, *x0)
+            pulled = jac.T @ ks_metric(a, p[1], p[2], p[3]) @ jac
+            want = bl_metric(a, r, th)
+            worst = max(worst, np.abs(pulled - want).max() / np.abs(want).max())
+    assert worst < 1e-8
+
+
+def launch(a, u, v, incl=60.0, dist=50.0):
+    origin = (C.c_double * 3)(dist * math.sin(math.radians(incl)), 0.0, dist * math.cos(math.radians(incl)))
+    fwd = -np.array(origin[:]) / dist
+    up = np.array([-math.cos(math.radians(incl)), 0.0, math.sin(math.radians(incl))])
+    right = np.cross(fwd, up)
+    d = (C.c_double * 3)(*(fwd + u * right + v * up))
+    st = O.KerrState()
+    O.lib.orc_kerr_launch(a, origin, d, C.byref(st))
+    return st
+
+
+def to_ks(a, st):
+    ks = O.KsState()
+    O.lib.orc_ks_from_bl(a, C.byref(st), C.byref(ks))
+    return ks
+
+
+def test_ks_launch_converts_back_and_is_null():
+    for u, v in ((0.05, 0.02), (0.0, 0.2), (-0.3, 0.1), (0.2, -0.25)):
+        st = launch(A, u, v)
+        ks = to_ks(A, st)
+        back = O.KerrState()
+        O.lib.orc_ks_to_bl(A, C.byref(ks), C.byref(back))
+        for f, _ in O.KerrState._fields_:
+            assert abs(getattr(back, f) - getattr(st, f)) < 1e-12 * max(1.0, abs(getattr(st, f)))
+        assert abs(O.lib.orc_ks_null_residual(A, C.byref(ks))) < 1e-10
+        assert abs(ks.x * ks.py - ks.y * ks.px - st.angular_momentum) < 1e-12
+
+
+def integrate(a, st, ks_form, step_factor=None, cap=400_000):
+    s = O.KerrSettings()
+    O.lib.orc_kerr_settings_default(C.byref(s), cap)
+    if step_factor:
+        s.step_factor = step_factor
+    res = O.KerrResult()
+    if ks_form:
+        O.lib.orc_ks_integrate(a, C.byref(to_ks(a, st)), C.byref(s), C.byref(res))
+    else:
+        O.lib.orc_kerr_integrate(a, C.byref(st), C.byref(s), C.byref(res))
+    d = (C.c_double * 3)()
+    O.lib.orc_kerr_direction(a, C.byref(res.state), d)
+    return res, np.array(d[:])
+
+
+def angle(x, y):
+    return math.acos(min(1.0, max(-1.0, float(x @ y))))
+
+
+def test_ks_agrees_with_boyer_lindquist_off_the_axis():
+    """Escaping rays whose axial angular momentum keeps them off the pole:
+    the same outcome and step count, escape directions within 2e-7 rad, and
+    captured rays captured by both."""
+    for u, v in ((0.1, -0.08), (0.2, 0.15), (0.3, 0.0), (-0.25, -0.1), (0.15, 0.3)):
+        bl, dbl = integrate(A, launch(A, u, v), False)
+        ks, dks = integrate(A, launch(A, u, v), True)
+        assert bl.outcome == ks.outcome == 1 and bl.steps == ks.steps
+        assert angle(dbl, dks) < 2e-7
+    bl, _ = integrate(A, launch(A, 0.05, 0.02), False)
+    ks, _ = integrate(A, launch(A, 0.05, 0.02), True)
+    assert bl.outcome == ks.outcome == 0
+
+
+def test_ks_at_zero_spin_matches_boyer_lindquist_schwarzschild():
+    for u, v in ((0.1, 0.0), (0.0, 0.15), (-0.2, 0.2), (0.04, 0.0)):
+        bl, dbl = integrate(0.0, launch(0.0, u, v), False, step_factor=0.005)
+        ks, dks = integrate(0.0, launch(0.0, u, v), True, step_factor=0.005)
+        assert bl.outcome == ks.outcome
+        if bl.outcome == 1:
+            assert angle(dbl, dks) < 1e-5
+
+
+def test_ks_converges_over_the_pole_where_boyer_lindquist_does_not():
+    """Rays aimed in the plane of the spin axis (the static observer gives
+    them only the small axial angular momentum of frame dragging), which
+    pass close over the pole: at the reference's step policy the Kerr-Schild
+    form lands within 1e-5 rad of its own converged direction (steps twenty
+    times finer) and keeps its drift gauges below 1e-7, while the
+    Boyer-Lindquist form at the same policy is ten times farther off and
+    drifts above 1e-5."""
+    for v in (0.12, 0.2, -0.1, 0.3):
+        st = launch(A, 0.0, v)
+        assert abs(st.angular_momentum) < 0.05
+        ks, dks = integrate(A, st, True)
+        fine, dfine = integrate(A, st, True, step_factor=0.001)
+        bl, dbl = integrate(A, st, False)
+        assert ks.outcome == fine.outcome == bl.outcome == 1
+        assert angle(dks, dfine) < 1e-5 and ks.drift < 1e-7
+        assert angle(dbl, dfine) > 10 * angle(dks, dfine) and bl.drift > 1e-5
