@@ -185,6 +185,22 @@ def main():
         fast_ms_per = fast_ms / max(3, args.steps // 2)
         plain["fast_integrator"] = {"ms_per_step": fast_ms_per, "value": fast_steps / (fast_ms_per * 1e-3),
                                     "steps_per_frame": fast_steps}
+        # The Kerr-Schild form (regular on the spin axis) with the plain
+        # launch: its cost, and its drift on the same frame.
+        plain_targets.reset_counters()
+        ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_KERR_SCHILD)
+        ks_totals = plain_targets.counters()
+        ks_debug = plain_targets.read()[1]
+        ks_kept = ks_debug[..., 2] != api.RAY_CAPTURED
+        ks_ms, _ = time_frames(
+            ctx, api, lambda: ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_KERR_SCHILD),
+            max(1, args.warmup // 2), max(3, args.steps // 2))
+        ks_ms_per = ks_ms / max(3, args.steps // 2)
+        plain["kerr_schild_integrator"] = {
+            "ms_per_step": ks_ms_per, "value": int(ks_totals.steps) / (ks_ms_per * 1e-3),
+            "steps_per_frame": int(ks_totals.steps),
+            "rays_over_budget_escaped_or_exhausted": int((ks_debug[..., 0][ks_kept] > u.drift_budget).sum()),
+            "max_drift_escaped_or_exhausted": float(ks_debug[..., 0][ks_kept].max())}
         plain_targets.close()
 
     achieved = steps_per_frame * FLOPS_PER_KERR_STEP / (ms_per_step * 1e-3) / 1e12

@@ -821,7 +821,7 @@ int garg_march_image(garg_ctx* ctx, garg_stream* stream, const garg_march_unifor
     REQUIRE(ctx && targets, "ctx and targets are required.");
     int rc = check_uniforms(uniforms);
     if (rc) return rc;
-    REQUIRE(integrator <= 1u, "unknown integrator %u.", integrator);
+    REQUIRE(integrator <= 2u, "unknown integrator %u.", integrator);
     REQUIRE(targets->width == uniforms->resolution[0] && targets->height == uniforms->resolution[1],
             "the targets are %ux%u but the uniforms ask for %ux%u.", targets->width, targets->height,
             uniforms->resolution[0], uniforms->resolution[1]);
@@ -850,7 +850,7 @@ int garg_probe(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes,
     REQUIRE(ctx && probes && results, "ctx, probes and results are required.");
     REFUSE_WHILE_CAPTURING(ctx, stream, "garg_probe");
     REQUIRE(count > 0, "no probes.");
-    REQUIRE(integrator <= 1u, "unknown integrator %u.", integrator);
+    REQUIRE(integrator <= 2u, "unknown integrator %u.", integrator);
     REQUIRE(uniforms && uniforms->spin >= 0.0f && uniforms->spin < 1.0f, "spin must be in [0, 1).");
     REQUIRE(uniforms->geodesic.step_cap <= (1u << 24), "the step cap is above 2^24.");
     REQUIRE(uniforms->spin > 0.0f || uniforms->sphere_radius <= 0.0f || tables,

@@ -855,6 +855,289 @@ struct KerrFastMarch {
     GARG_DEV f3 final_position() const { return kerr_position_fast(k.a2, r, e.st, e.ct, ph); }
 };
 
+// ---------------------------------------------------------------------------
+// Kerr in Cartesian Kerr-Schild coordinates: the integrator form the
+// reference names as future work for the polar seam of its Boyer-Lindquist
+// march (docs/lab/07-kerr.md:33-45). The metric g = eta + f l l, with
+// f = 2 r^3 / (r^4 + a^2 z^2) and l = (1, (r x + a y) / (r^2 + a^2),
+// (r y - a x) / (r^2 + a^2), z / r), is regular on the spin axis, so rays
+// that pass over the pole need neither the clamped 1/sin^2 terms nor the
+// pole factor of the step length. A photon with p_t = -E moves by
+//   dx/dlambda = p - f (E + l.p) l,
+//   dp/dlambda = (1/2) (E + l.p)^2 grad f + f (E + l.p) (grad l) p,
+// in the affine parameter of the Boyer-Lindquist form. Rays launch from the
+// reference's static observer (kerr_launch, converted), and the gas, the
+// path lengths and the sky are taken at the Boyer-Lindquist position and
+// direction of each point, so the image is the reference's wherever its
+// coordinates are regular.
+
+struct KsState {
+    float x, y, z, px, py, pz;
+};
+
+struct KsConsts {
+    float a, a2, E, rplus, phase;  // phase: int a/Delta dr at the launch radius
+    float rp, rm, shift_scale;     // horizons and a / (r+ - r-)
+};
+
+GARG_DEV float ks_radius(float a2, float x, float y, float z) {
+    float half = 0.5f * (((x * x + y * y) + z * z) - a2);
+    return __builtin_amdgcn_sqrtf(half + __builtin_amdgcn_sqrtf(half * half + a2 * (z * z)));
+}
+
+// int a / Delta dr up to a constant: phi~ - phi.
+GARG_DEV float ks_phi_shift(const KsConsts& k, float r) {
+    return k.shift_scale * __logf(fabsf((r - k.rp) / (r - k.rm)));
+}
+
+// Hamilton's equations at a state.
+GARG_DEV KsState ks_derivative(const KsConsts& k, const KsState& s) {
+    const float a = k.a, a2 = k.a2;
+    const float r = ks_radius(a2, s.x, s.y, s.z);
+    const float r2 = r * r, ir = __builtin_amdgcn_rcpf(r);
+    const float q = r2 + a2, iq = __builtin_amdgcn_rcpf(q);
+    const float d = r2 * r2 + a2 * (s.z * s.z), id = __builtin_amdgcn_rcpf(d);
+    const float r3 = r2 * r;
+    const float f = 2.0f * r3 * id;
+    const float lx = (r * s.x + a * s.y) * iq, ly = (r * s.y - a * s.x) * iq, lz = s.z * ir;
+    const float big = k.E + ((lx * s.px + ly * s.py) + lz * s.pz);
+    const float fb = f * big;
+    KsState out;
+    out.x = s.px - fb * lx;
+    out.y = s.py - fb * ly;
+    out.z = s.pz - fb * lz;
+    const float drx = s.x * r3 * id, dry = s.y * r3 * id, drz = s.z * r * q * id;
+    const float id2 = id * id;
+    const float fr = 2.0f * r2 * (3.0f * a2 * (s.z * s.z) - r2 * r2) * id2;
+    const float fz = -4.0f * a2 * s.z * r3 * id2;
+    const float iq2 = iq * iq;
+    const float lrx = (s.x * (a2 - r2) - 2.0f * a * r * s.y) * iq2;
+    const float lry = (s.y * (a2 - r2) + 2.0f * a * r * s.x) * iq2;
+    const float lrz = -s.z * ir * ir;
+    const float A = (lrx * s.px + lry * s.py) + lrz * s.pz;
+    const float hb = 0.5f * big * big;
+    out.px = hb * (fr * drx) + fb * (A * drx + (r * s.px - a * s.py) * iq);
+    out.py = hb * (fr * dry) + fb * (A * dry + (a * s.px + r * s.py) * iq);
+    out.pz = hb * (fr * drz + fz) + fb * (A * drz + s.pz * ir);
+    return out;
+}
+
+GARG_DEV KsState ks_axpy(const KsState& s, const KsState& k, float h) {
+    return KsState{s.x + k.x * h, s.y + k.y * h, s.z + k.z * h, s.px + k.px * h, s.py + k.py * h, s.pz + k.pz * h};
+}
+
+// The reference's static-observer launch (Boyer-Lindquist covariant
+// momenta) in Kerr-Schild coordinates: raise with the Boyer-Lindquist
+// inverse metric, map the vector through the coordinate change (phi~ = phi
+// at the launch), lower with the Kerr-Schild metric. Once per ray, in
+// precise arithmetic.
+GARG_DEV KsState ks_from_bl(const KsConsts& k, const KerrRayState& s) {
+    const float a = k.a, a2 = k.a2, r = s.r;
+    float st, ct, sp, cp;
+    sincosf(s.theta, &st, &ct);
+    sincosf(s.phi, &sp, &cp);
+    const float sin2 = fmaxf(st * st, 1e-12f);
+    const float delta = r * r - 2.0f * r + a2, sigma = r * r + a2 * ct * ct;
+    const float E = s.energy, L = s.angular_momentum;
+    const float g_tt = -((r * r + a2) * (r * r + a2) - a2 * delta * sin2) / (sigma * delta);
+    const float g_tphi = -2.0f * a * r / (sigma * delta);
+    const float g_phiphi = (delta - a2 * sin2) / (sigma * delta * sin2);
+    const float kt = g_tt * (-E) + g_tphi * L;
+    const float kphi = g_tphi * (-E) + g_phiphi * L;
+    const float kr = delta * s.pr / sigma, kth = s.ptheta / sigma;
+    const float kt_ks = kt + 2.0f * r / delta * kr;
+    const float kphi_ks = kphi + a / delta * kr;
+    const float x = (r * cp - a * sp) * st, y = (r * sp + a * cp) * st, z = r * ct;
+    const float vx = cp * st * kr + (r * cp - a * sp) * ct * kth - y * kphi_ks;
+    const float vy = sp * st * kr + (r * sp + a * cp) * ct * kth + x * kphi_ks;
+    const float vz = ct * kr - r * st * kth;
+    const float rr = ks_radius(a2, x, y, z);
+    const float q = rr * rr + a2;
+    const float f = 2.0f * rr * rr * rr / (rr * rr * rr * rr + a2 * z * z);
+    const float lx = (rr * x + a * y) / q, ly = (rr * y - a * x) / q, lz = z / rr;
+    const float lk = kt_ks + (lx * vx + ly * vy) + lz * vz;
+    return KsState{x, y, z, vx + f * lx * lk, vy + f * ly * lk, vz + f * lz * lk};
+}
+
+struct KerrSchildMarch {
+    KsConsts k;
+    KsState s, c;   // state and TwoSum carries
+    float r;        // radius of the current point
+    float carter0, l0, inv_scale, inv_root_scale, capture;
+    Gather g;
+    f3 prev;        // Boyer-Lindquist-convention position of the previous point
+    float drift;
+    uint32_t steps;
+    uint32_t outcome;
+    bool done;
+
+    GARG_DEV void begin(float spin, const KerrRayState& launched) {
+        k.a = spin;
+        k.a2 = spin * spin;
+        k.E = launched.energy;
+        const float root = sqrtf(fmaxf(1.0f - k.a2, 0.0f));
+        k.rp = 1.0f + root;
+        k.rm = 1.0f - root;
+        k.rplus = k.rp;
+        k.shift_scale = root > 0.0f ? spin / (k.rp - k.rm) : 0.0f;
+        k.phase = 0.0f;
+        k.phase = ks_phi_shift(k, launched.r);
+        s = ks_from_bl(k, launched);
+        c = KsState{0, 0, 0, 0, 0, 0};
+        r = ks_radius(k.a2, s.x, s.y, s.z);
+        l0 = s.x * s.py - s.y * s.px;
+        carter0 = carter_of(s, r);
+        const float scale = fmaxf(fabsf(carter0), k.E * k.E);
+        inv_scale = 1.0f / scale;
+        inv_root_scale = 1.0f / sqrtf(scale);
+        capture = k.rplus + GARG_F(KERR_CAPTURE_MARGIN);
+        g.reset();
+        prev = position_bl();
+        drift = 0.0f;
+        steps = 0;
+        outcome = GARG_RAY_EXHAUSTED;
+        done = false;
+    }
+
+    // The Carter constant as the Boyer-Lindquist form writes it,
+    // p_theta^2 + cos^2(theta) (L^2 / sin^2(theta) - a^2 E^2), with
+    // L = x p_y - y p_x, sin^2(theta) = rho^2 / (r^2 + a^2) and
+    // p_theta = (cos(theta) (r^2 + a^2) (x p_x + y p_y) - r rho^2 p_z)
+    // / (rho sqrt(r^2 + a^2)). Near the axis, 1 - cos^2(theta) would lose
+    // the small sin^2(theta) that L^2 / sin^2(theta) divides by.
+    GARG_DEV float carter_of(const KsState& t, float rr) const {
+        const float q = rr * rr + k.a2;
+        const float rho2 = fmaxf(t.x * t.x + t.y * t.y, 1e-30f);
+        const float ct = t.z / rr;
+        const float pth = (ct * q * (t.x * t.px + t.y * t.py) - rr * rho2 * t.pz) * __builtin_amdgcn_rsqf(rho2 * q);
+        const float L = t.x * t.py - t.y * t.px;
+        return pth * pth + ct * ct * (L * L * q / rho2 - k.a2 * k.E * k.E);
+    }
+
+    // The position in the renderer's convention (sqrt(r^2 + a^2) sin(theta)
+    // (cos phi, sin phi), r cos(theta), Boyer-Lindquist phi): the
+    // Kerr-Schild point turned about z by atan2(a, r) + (phi~ - phi).
+    GARG_DEV f3 position_bl() const {
+        const float turn = atan2f(k.a, r) + (ks_phi_shift(k, r) - k.phase);
+        float sn, cs;
+        fast_sincos(turn, &sn, &cs);
+        return f3{cs * s.x + sn * s.y, cs * s.y - sn * s.x, s.z};
+    }
+
+    // The locally measured direction in the Boyer-Lindquist frame, from the
+    // coordinate velocities (kerr_direction's convention).
+    GARG_DEV f3 direction_bl() const {
+        const KsState v = ks_derivative(k, s);
+        const float r2 = r * r, q = r2 + k.a2;
+        const float d = r2 * r2 + k.a2 * s.z * s.z;
+        const float rdot = (s.x * v.x + s.y * v.y) * r2 * r / d + s.z * v.z * r * q / d;
+        const float ct = fminf(fmaxf(s.z / r, -1.0f), 1.0f);
+        const float st = sqrtf(fmaxf(1.0f - ct * ct, 1e-12f));
+        const float rho2 = fmaxf(s.x * s.x + s.y * s.y, 1e-30f);
+        // theta' from z = r cos(theta) away from the axis, from
+        // rho = sqrt(r^2 + a^2) sin(theta) near it, where the first divides
+        // two small numbers.
+        float thdot;
+        if (fabsf(ct) < 0.7f) {
+            thdot = (ct * rdot - v.z) / (r * st);
+        } else {
+            const float rq = sqrtf(q);
+            const float rhodot = (s.x * v.x + s.y * v.y) * __builtin_amdgcn_rsqf(rho2);
+            thdot = (rhodot - r * rdot * st / rq) / (rq * ct);
+        }
+        const float delta = r2 - 2.0f * r + k.a2;
+        const float phdot = (s.x * v.y - s.y * v.x) / rho2 + k.a * rdot / q - k.a * rdot / delta;
+        const float phi = atan2f(s.y, s.x) - atan2f(k.a, r) - (ks_phi_shift(k, r) - k.phase);
+        float sp, cp;
+        fast_sincos(phi, &sp, &cp);
+        const f3 r_hat{st * cp, st * sp, ct};
+        const f3 theta_hat{ct * cp, ct * sp, -st};
+        const f3 phi_hat{-sp, cp, 0.0f};
+        const f3 local = r_hat * rdot + theta_hat * (r * thdot) + phi_hat * (r * st * phdot);
+        const float len2 = dot(local, local);
+        return len2 > 0.0f ? local * __builtin_amdgcn_rsqf(len2) : r_hat;
+    }
+
+    GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget) {
+        const garg_geodesic_settings& gs = u.geodesic;
+        const bool has_volume = gather && sc.volume.voxels != nullptr;
+        for (uint32_t n = 0; n < budget && !done; ++n) {
+            if (steps >= gs.step_cap) {
+                done = true;
+                break;
+            }
+            // The Boyer-Lindquist step policy without its pole factor.
+            const float h = fminf(fmaxf(fminf(gs.step_factor * r, 0.2f * (r - k.rplus)), GARG_F(KERR_STEP_FLOOR)),
+                                  gs.step_max);
+            const float hh = 0.5f * h;
+            const KsState k1 = ks_derivative(k, s);
+            const KsState k2 = ks_derivative(k, ks_axpy(s, k1, hh));
+            const KsState k3 = ks_derivative(k, ks_axpy(s, k2, hh));
+            const KsState k4 = ks_derivative(k, ks_axpy(s, k3, h));
+            const float w = h * (1.0f / 6.0f);
+            two_sum(s.x, c.x, ((k1.x + k2.x * 2.0f) + (k3.x * 2.0f + k4.x)) * w);
+            two_sum(s.y, c.y, ((k1.y + k2.y * 2.0f) + (k3.y * 2.0f + k4.y)) * w);
+            two_sum(s.z, c.z, ((k1.z + k2.z * 2.0f) + (k3.z * 2.0f + k4.z)) * w);
+            two_sum(s.px, c.px, ((k1.px + k2.px * 2.0f) + (k3.px * 2.0f + k4.px)) * w);
+            two_sum(s.py, c.py, ((k1.py + k2.py * 2.0f) + (k3.py * 2.0f + k4.py)) * w);
+            two_sum(s.pz, c.pz, ((k1.pz + k2.pz * 2.0f) + (k3.pz * 2.0f + k4.pz)) * w);
+            if (!compensated) c = KsState{0, 0, 0, 0, 0, 0};
+            steps += 1;
+            r = ks_radius(k.a2, s.x, s.y, s.z);
+            // Drift gauges: the null residual 2 Sigma H everywhere, the Carter
+            // constant away from the axis (its formula divides by sin^2),
+            // the axial angular momentum everywhere.
+            {
+                const float q = r * r + k.a2, d = r * r * r * r + k.a2 * s.z * s.z;
+                const float f = 2.0f * r * r * r / d;
+                const float lx = (r * s.x + k.a * s.y) / q, ly = (r * s.y - k.a * s.x) / q, lz = s.z / r;
+                const float big = k.E + ((lx * s.px + ly * s.py) + lz * s.pz);
+                const float hamilton =
+                    0.5f * ((((s.px * s.px + s.py * s.py) + s.pz * s.pz) - k.E * k.E) - f * big * big);
+                const float sigma = r * r + k.a2 * (s.z * s.z) / (r * r);
+                float dn = fabsf(2.0f * sigma * hamilton) * inv_scale / fmaxf(r * r, 1.0f);
+                if (s.x * s.x + s.y * s.y > 1e-4f * (r * r + k.a2))
+                    dn = fmaxf(dn, fabsf(carter_of(s, r) - carter0) * inv_scale);
+                dn = fmaxf(dn, fabsf((s.x * s.py - s.y * s.px) - l0) * inv_root_scale);
+                drift = fmaxf(drift, dn);
+            }
+            const f3 p = position_bl();
+            const f3 dp = p - prev;
+            const float path = __builtin_amdgcn_sqrtf(dot(dp, dp));
+            g.travelled += path;
+            if (r <= capture) {
+                outcome = GARG_RAY_CAPTURED;
+                done = true;
+                break;
+            }
+            if (has_volume) {
+                const float sigma = r * r + k.a2 * (s.z * s.z) / (r * r);
+                const float f = 1.0f - 2.0f * r / sigma;
+                g.sample(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, fmaxf(f, 0.0f),
+                         [&]() { return direction_bl(); });
+            }
+            prev = p;
+            if (r >= gs.escape_radius) {
+                outcome = GARG_RAY_ESCAPED;
+                done = true;
+                break;
+            }
+        }
+        if (!done && steps >= gs.step_cap) done = true;
+    }
+
+    // Whether the ray falls into the hole (outward speed negative when the
+    // budget ran out, as the Boyer-Lindquist form reads p_r).
+    GARG_DEV bool falls_in() const {
+        if (outcome == GARG_RAY_CAPTURED) return true;
+        if (outcome != GARG_RAY_EXHAUSTED) return false;
+        const KsState v = ks_derivative(k, s);
+        return (s.x * v.x + s.y * v.y) * r * r + s.z * v.z * (r * r + k.a2) < 0.0f;
+    }
+    GARG_DEV f3 final_direction() const { return direction_bl(); }
+    GARG_DEV f3 final_position() const { return position_bl(); }
+};
+
 // Colour of a finished ray: gas plus the starfield behind it unless it falls
 // in (march.metal:226-229, 261-263).
 GARG_DEV f3 finish_color(const Gather& g, bool falls_in, f3 sky, const garg_march_uniforms& u, const Scene& sc) {

@@ -33,6 +33,15 @@ GARG_DEV PixelResult shade_pixel(const garg_march_uniforms& u, const Scene& sc, 
             out.steps = m.steps;
             out.outcome = m.outcome;
             out.depth = fminf(m.g.depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f);
+        } else if (Form == GARG_INTEGRATOR_KERR_SCHILD) {
+            KerrSchildMarch m;
+            m.begin(u.spin, launched);
+            m.run(u, sc, true, u.compensated != 0u, cap);
+            out.color = finish_color(m.g, m.falls_in(), m.final_direction(), u, sc);
+            out.drift = m.drift;
+            out.steps = m.steps;
+            out.outcome = m.outcome;
+            out.depth = fminf(m.g.depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f);
         } else {
             KerrFastMarch m;
             m.begin(u.spin, launched);
@@ -55,9 +64,9 @@ GARG_DEV PixelResult shade_pixel(const garg_march_uniforms& u, const Scene& sc, 
         out.depth = 1.0f;
         return out;
     }
-    SchwMarchT<Form == GARG_INTEGRATOR_FAST> m;
+    SchwMarchT<Form != GARG_INTEGRATOR_REFERENCE> m;
     m.begin(p.ray);
-    m.run(u, sc, true, Form == GARG_INTEGRATOR_FAST && u.compensated != 0u, cap);
+    m.run(u, sc, true, Form != GARG_INTEGRATOR_REFERENCE && u.compensated != 0u, cap);
     bool in = m.falls_in();
     out.color = finish_color(m.g, in, in ? f3{0, 0, 0} : m.sky_direction(u, sc), u, sc);
     out.drift = ray_drift(m.ray);
@@ -107,6 +116,15 @@ __global__ __launch_bounds__(64) void probe_kernel(ProbeArgs a) {
             out.drift = m.drift;
             out.outcome = m.outcome;
             out.steps = m.steps;
+        } else if (Form == GARG_INTEGRATOR_KERR_SCHILD) {
+            KerrSchildMarch m;
+            m.begin(u.spin, launched);
+            m.run(u, sc, false, u.compensated != 0u, u.geodesic.step_cap);
+            pos = m.final_position();
+            d = m.final_direction();
+            out.drift = m.drift;
+            out.outcome = m.outcome;
+            out.steps = m.steps;
         } else {
             KerrFastMarch m;
             m.begin(u.spin, launched);
@@ -130,9 +148,9 @@ __global__ __launch_bounds__(64) void probe_kernel(ProbeArgs a) {
             out.energy = p.ray.energy;
             out.angular_momentum = p.ray.angular_momentum;
         } else {
-            SchwMarchT<Form == GARG_INTEGRATOR_FAST> m;
+            SchwMarchT<Form != GARG_INTEGRATOR_REFERENCE> m;
             m.begin(p.ray);
-            m.run(u, sc, false, Form == GARG_INTEGRATOR_FAST && u.compensated != 0u, u.geodesic.step_cap);
+            m.run(u, sc, false, Form != GARG_INTEGRATOR_REFERENCE && u.compensated != 0u, u.geodesic.step_cap);
             pos = m.position();
             d = (m.outcome == GARG_RAY_ESCAPED && !m.falls_in()) ? m.sky_direction(u, sc) : m.direction();
             out.drift = ray_drift(m.ray);
@@ -155,6 +173,8 @@ hipError_t launch_march_image(const MarchImageArgs& a, uint32_t integrator, hipS
     dim3 grid((a.extent_w + 7u) / 8u, (a.extent_h + 7u) / 8u);
     if (integrator == GARG_INTEGRATOR_REFERENCE)
         march_image_kernel<GARG_INTEGRATOR_REFERENCE><<<grid, 64, 0, stream>>>(a);
+    else if (integrator == GARG_INTEGRATOR_KERR_SCHILD)
+        march_image_kernel<GARG_INTEGRATOR_KERR_SCHILD><<<grid, 64, 0, stream>>>(a);
     else
         march_image_kernel<GARG_INTEGRATOR_FAST><<<grid, 64, 0, stream>>>(a);
     return hipGetLastError();
@@ -164,6 +184,8 @@ hipError_t launch_probe(const ProbeArgs& a, uint32_t integrator, hipStream_t str
     dim3 grid((a.count + 63u) / 64u);
     if (integrator == GARG_INTEGRATOR_REFERENCE)
         probe_kernel<GARG_INTEGRATOR_REFERENCE><<<grid, 64, 0, stream>>>(a);
+    else if (integrator == GARG_INTEGRATOR_KERR_SCHILD)
+        probe_kernel<GARG_INTEGRATOR_KERR_SCHILD><<<grid, 64, 0, stream>>>(a);
     else
         probe_kernel<GARG_INTEGRATOR_FAST><<<grid, 64, 0, stream>>>(a);
     return hipGetLastError();

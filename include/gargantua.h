@@ -51,8 +51,13 @@ enum { GARG_GENERATOR_DISK = 0, GARG_GENERATOR_ANNULUS = 1 };
  * reciprocals per stage, drift gauges shared with the next stage;
  * Schwarzschild: one reciprocal of r per stage in place of four divisions),
  * with TwoSum-compensated state accumulation when the uniforms ask for it.
+ * KERR_SCHILD integrates Kerr rays in Cartesian Kerr-Schild coordinates,
+ * whose metric is regular on the spin axis: the form the reference names as
+ * future work for its polar seam (docs/lab/07-kerr.md:33-45). It takes the
+ * same launch, step policy (without the pole factor), gas and sky as the
+ * Boyer-Lindquist forms; at spin 0 it is the FAST Schwarzschild form.
  * The batched march always runs the FAST form. */
-enum { GARG_INTEGRATOR_REFERENCE = 0, GARG_INTEGRATOR_FAST = 1 };
+enum { GARG_INTEGRATOR_REFERENCE = 0, GARG_INTEGRATOR_FAST = 1, GARG_INTEGRATOR_KERR_SCHILD = 2 };
 
 typedef struct garg_ctx garg_ctx;
 typedef struct garg_stream garg_stream;
