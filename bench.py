@@ -202,6 +202,24 @@ def main():
             "rays_over_budget_escaped_or_exhausted": int((ks_debug[..., 0][ks_kept] > u.drift_budget).sum()),
             "max_drift_escaped_or_exhausted": float(ks_debug[..., 0][ks_kept].max())}
         plain_targets.close()
+        # The same batched launch running the Kerr-Schild form.
+        ks_cfg = ctx.batch_config(block=cfg.block, waves_per_simd=cfg.waves_per_simd,
+                                  compaction_period=cfg.compaction_period, refill_threshold=cfg.refill_threshold,
+                                  kerr_schild=1)
+        targets.reset_counters()
+        ks_batch = ctx.march_batch(request, ks_cfg)
+        ks_frame = targets.read()[1]
+        ks_frame_kept = ks_frame[..., 2] != api.RAY_CAPTURED
+        ks_batched_ms, ks_frames = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, request, ks_cfg),
+                                               max(1, args.warmup // 2), max(3, args.steps // 2))
+        ks_batched_per = ks_batched_ms / max(3, args.steps // 2)
+        plain["kerr_schild_batched"] = {
+            "ms_per_step": ks_batched_per, "value": int(ks_batch.steps) / (ks_batched_per * 1e-3),
+            "steps_per_frame": int(ks_batch.steps),
+            "speedup_vs_plain": plain_ms_per / ks_batched_per,
+            "rays_over_budget_escaped_or_exhausted": int((ks_frame[..., 0][ks_frame_kept] > u.drift_budget).sum()),
+            "max_drift_escaped_or_exhausted": float(ks_frame[..., 0][ks_frame_kept].max()),
+            "p50_ms": float(np.percentile(ks_frames, 50)), "p99_ms": float(np.percentile(ks_frames, 99))}
 
     achieved = steps_per_frame * FLOPS_PER_KERR_STEP / (ms_per_step * 1e-3) / 1e12
     line = {

@@ -82,7 +82,8 @@ class MarchRequest(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    _fields_ = [("block", u32), ("waves_per_simd", u32), ("compaction_period", u32), ("refill_threshold", u32)]
+    _fields_ = [("block", u32), ("waves_per_simd", u32), ("compaction_period", u32), ("refill_threshold", u32),
+                ("kerr_schild", u32)]
 
 
 # Every entry point of include/gargantua.h with its argument types.

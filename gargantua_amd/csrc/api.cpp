@@ -890,6 +890,7 @@ int garg_batch_config_default(garg_ctx* ctx, garg_batch_config* out) {
     out->waves_per_simd = 5;
     out->compaction_period = 32;
     out->refill_threshold = 0;
+    out->kerr_schild = 0;
     return GARG_OK;
 }
 
@@ -906,6 +907,7 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
     REQUIRE(config->block == 64 || config->block == 128 || config->block == 256, "block must be 64, 128 or 256.");
     REQUIRE(config->compaction_period > 0, "the compaction period must be positive.");
     REQUIRE(config->refill_threshold <= 64, "the refill threshold is a lane count of at most 64.");
+    REQUIRE(config->kerr_schild <= 1u, "kerr_schild is 0 or 1, not %u.", config->kerr_schild);
     for (uint32_t i = 0; i < count; ++i) {
         const garg_march_request& r = requests[i];
         int rc = check_uniforms(&r.uniforms);
@@ -987,13 +989,13 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
         BatchArgs a;
         a.requests = table;
         a.request_count = n;
-        a.requests_kerr = kerr ? 1u : 0u;
+        a.kind = !kerr ? BATCH_SCHWARZSCHILD : config->kerr_schild ? BATCH_KERR_SCHILD : BATCH_KERR;
         a.total_rays = queue;
         a.queue = ctx->queue + (size_t)part * 8 * 16;
         a.totals = ctx->batch_totals;
         a.compaction_period = config->compaction_period;
         a.refill_threshold = config->refill_threshold;
-        uint32_t resident = march_batch_resident_blocks(kerr, config->refill_threshold == 0u, config->block);
+        uint32_t resident = march_batch_resident_blocks(a.kind, config->refill_threshold == 0u, config->block);
         uint32_t launch_grid = resident ? std::min(grid, resident * ctx->info.compute_units) : grid;
         HIP_TRY(launch_march_batch(a, launch_grid, config->block, s), "launching the batched march");
     }

@@ -876,8 +876,9 @@ struct KsState {
 };
 
 struct KsConsts {
-    float a, a2, E, rplus, phase;  // phase: int a/Delta dr at the launch radius
-    float rp, rm, shift_scale;     // horizons and a / (r+ - r-)
+    float a, a2, E, rplus;
+    float rp, rm, shift_scale;  // horizons and a / (r+ - r-)
+    float phase;                // ln |(r - r+) / (r - r-)| at the launch radius
 };
 
 GARG_DEV float ks_radius(float a2, float x, float y, float z) {
@@ -885,28 +886,41 @@ GARG_DEV float ks_radius(float a2, float x, float y, float z) {
     return __builtin_amdgcn_sqrtf(half + __builtin_amdgcn_sqrtf(half * half + a2 * (z * z)));
 }
 
-// int a / Delta dr up to a constant: phi~ - phi.
-GARG_DEV float ks_phi_shift(const KsConsts& k, float r) {
-    return k.shift_scale * __logf(fabsf((r - k.rp) / (r - k.rm)));
-}
+GARG_DEV float ks_log_ratio(const KsConsts& k, float r) { return __logf(fabsf((r - k.rp) / (r - k.rm))); }
 
-// Hamilton's equations at a state.
-GARG_DEV KsState ks_derivative(const KsConsts& k, const KsState& s) {
+// Hamilton's equations at a state, with the pieces of the metric that the
+// drift gauges reuse.
+struct KsEval {
+    KsState d;
+    float r, ir, q, f, big;
+};
+
+GARG_DEV KsEval ks_eval(const KsConsts& k, const KsState& s) {
     const float a = k.a, a2 = k.a2;
-    const float r = ks_radius(a2, s.x, s.y, s.z);
-    const float r2 = r * r, ir = __builtin_amdgcn_rcpf(r);
-    const float q = r2 + a2, iq = __builtin_amdgcn_rcpf(q);
-    const float d = r2 * r2 + a2 * (s.z * s.z), id = __builtin_amdgcn_rcpf(d);
+    KsEval e;
+    // r^2 = half + sqrt(half^2 + a^2 z^2); r and 1/r from one reciprocal
+    // square root, 1/(r^2 + a^2) and 1/d from one reciprocal.
+    const float half = 0.5f * (((s.x * s.x + s.y * s.y) + s.z * s.z) - a2);
+    const float rr2 = half + __builtin_amdgcn_sqrtf(half * half + a2 * (s.z * s.z));
+    const float ir = __builtin_amdgcn_rsqf(rr2);
+    const float r = rr2 * ir;
+    const float r2 = r * r;
+    const float q = r2 + a2;
+    const float d = r2 * r2 + a2 * (s.z * s.z);
+    const float iqd = __builtin_amdgcn_rcpf(q * d);
+    const float iq = d * iqd, id = q * iqd;
     const float r3 = r2 * r;
     const float f = 2.0f * r3 * id;
     const float lx = (r * s.x + a * s.y) * iq, ly = (r * s.y - a * s.x) * iq, lz = s.z * ir;
     const float big = k.E + ((lx * s.px + ly * s.py) + lz * s.pz);
     const float fb = f * big;
-    KsState out;
-    out.x = s.px - fb * lx;
-    out.y = s.py - fb * ly;
-    out.z = s.pz - fb * lz;
-    const float drx = s.x * r3 * id, dry = s.y * r3 * id, drz = s.z * r * q * id;
+    e.d.x = s.px - fb * lx;
+    e.d.y = s.py - fb * ly;
+    e.d.z = s.pz - fb * lz;
+    // grad r = (x r^3, y r^3, z r (r^2 + a^2)) / d. With f_r = df/dr at
+    // fixed z and A = (dl/dr at fixed x, y, z) . p, the momentum equation
+    // is (1/2) big^2 (f_r grad r + f_z e_z) + f big (A grad r + (dl/dx)|_r p),
+    // so the grad r terms share one factor.
     const float id2 = id * id;
     const float fr = 2.0f * r2 * (3.0f * a2 * (s.z * s.z) - r2 * r2) * id2;
     const float fz = -4.0f * a2 * s.z * r3 * id2;
@@ -916,10 +930,17 @@ GARG_DEV KsState ks_derivative(const KsConsts& k, const KsState& s) {
     const float lrz = -s.z * ir * ir;
     const float A = (lrx * s.px + lry * s.py) + lrz * s.pz;
     const float hb = 0.5f * big * big;
-    out.px = hb * (fr * drx) + fb * (A * drx + (r * s.px - a * s.py) * iq);
-    out.py = hb * (fr * dry) + fb * (A * dry + (a * s.px + r * s.py) * iq);
-    out.pz = hb * (fr * drz + fz) + fb * (A * drz + s.pz * ir);
-    return out;
+    const float g = (hb * fr + fb * A) * id;
+    const float gxy = g * r3, fbq = fb * iq;
+    e.d.px = gxy * s.x + fbq * (r * s.px - a * s.py);
+    e.d.py = gxy * s.y + fbq * (a * s.px + r * s.py);
+    e.d.pz = g * (s.z * r * q) + (hb * fz + fb * s.pz * ir);
+    e.r = r;
+    e.ir = ir;
+    e.q = q;
+    e.f = f;
+    e.big = big;
+    return e;
 }
 
 GARG_DEV KsState ks_axpy(const KsState& s, const KsState& k, float h) {
@@ -959,13 +980,22 @@ GARG_DEV KsState ks_from_bl(const KsConsts& k, const KerrRayState& s) {
     return KsState{x, y, z, vx + f * lx * lk, vy + f * ly * lk, vz + f * lz * lk};
 }
 
+// The renderer draws Boyer-Lindquist positions, sqrt(r^2 + a^2)
+// sin(theta) (cos phi, sin phi) and r cos(theta): the Kerr-Schild point
+// turned about z by -psi(r), psi = atan2(a, r) + (phi~ - phi)(r), whose
+// second term is (a / (r+ - r-)) ln |(r - r+) / (r - r-)| less its launch
+// value. The march tracks only how psi changes from step to step (a
+// small-angle atan and a log1p of the exact differences, no atan2), which
+// is what the length of a step's chord needs; the absolute rotation is
+// formed only where the gas is sampled and at the end of the ray.
 struct KerrSchildMarch {
     KsConsts k;
     KsState s, c;   // state and TwoSum carries
-    float r;        // radius of the current point
+    KsEval e;       // evaluation at the current state
     float carter0, l0, inv_scale, inv_root_scale, capture;
     Gather g;
-    f3 prev;        // Boyer-Lindquist-convention position of the previous point
+    f3 prev;        // Kerr-Schild position of the previous point
+    float prev_r;
     float drift;
     uint32_t steps;
     uint32_t outcome;
@@ -980,19 +1010,19 @@ struct KerrSchildMarch {
         k.rm = 1.0f - root;
         k.rplus = k.rp;
         k.shift_scale = root > 0.0f ? spin / (k.rp - k.rm) : 0.0f;
-        k.phase = 0.0f;
-        k.phase = ks_phi_shift(k, launched.r);
+        k.phase = ks_log_ratio(k, launched.r);
         s = ks_from_bl(k, launched);
         c = KsState{0, 0, 0, 0, 0, 0};
-        r = ks_radius(k.a2, s.x, s.y, s.z);
+        e = ks_eval(k, s);
         l0 = s.x * s.py - s.y * s.px;
-        carter0 = carter_of(s, r);
+        carter0 = carter();
         const float scale = fmaxf(fabsf(carter0), k.E * k.E);
         inv_scale = 1.0f / scale;
         inv_root_scale = 1.0f / sqrtf(scale);
         capture = k.rplus + GARG_F(KERR_CAPTURE_MARGIN);
         g.reset();
-        prev = position_bl();
+        prev = f3{s.x, s.y, s.z};
+        prev_r = e.r;
         drift = 0.0f;
         steps = 0;
         outcome = GARG_RAY_EXHAUSTED;
@@ -1005,35 +1035,42 @@ struct KerrSchildMarch {
     // p_theta = (cos(theta) (r^2 + a^2) (x p_x + y p_y) - r rho^2 p_z)
     // / (rho sqrt(r^2 + a^2)). Near the axis, 1 - cos^2(theta) would lose
     // the small sin^2(theta) that L^2 / sin^2(theta) divides by.
-    GARG_DEV float carter_of(const KsState& t, float rr) const {
-        const float q = rr * rr + k.a2;
-        const float rho2 = fmaxf(t.x * t.x + t.y * t.y, 1e-30f);
-        const float ct = t.z / rr;
-        const float pth = (ct * q * (t.x * t.px + t.y * t.py) - rr * rho2 * t.pz) * __builtin_amdgcn_rsqf(rho2 * q);
-        const float L = t.x * t.py - t.y * t.px;
-        return pth * pth + ct * ct * (L * L * q / rho2 - k.a2 * k.E * k.E);
+    GARG_DEV float carter() const {
+        const float rho2 = fmaxf(s.x * s.x + s.y * s.y, 1e-30f);
+        const float ct = s.z * e.ir;
+        const float pth = (ct * e.q * (s.x * s.px + s.y * s.py) - e.r * rho2 * s.pz) * __builtin_amdgcn_rsqf(rho2 * e.q);
+        const float L = s.x * s.py - s.y * s.px;
+        return pth * pth + ct * ct * (L * L * e.q * __builtin_amdgcn_rcpf(rho2) - k.a2 * k.E * k.E);
     }
 
-    // The position in the renderer's convention (sqrt(r^2 + a^2) sin(theta)
-    // (cos phi, sin phi), r cos(theta), Boyer-Lindquist phi): the
-    // Kerr-Schild point turned about z by atan2(a, r) + (phi~ - phi).
+    // cos and sin of psi at the current point.
+    GARG_DEV void rotation(float* cpsi, float* spsi) const {
+        const float rq = __builtin_amdgcn_rsqf(e.q);
+        const float ca = e.r * rq, sa = k.a * rq;
+        float sx, cx;
+        fast_sincos(k.shift_scale * (ks_log_ratio(k, e.r) - k.phase), &sx, &cx);
+        *cpsi = ca * cx - sa * sx;
+        *spsi = sa * cx + ca * sx;
+    }
+
     GARG_DEV f3 position_bl() const {
-        const float turn = atan2f(k.a, r) + (ks_phi_shift(k, r) - k.phase);
-        float sn, cs;
-        fast_sincos(turn, &sn, &cs);
+        float cs, sn;
+        rotation(&cs, &sn);
         return f3{cs * s.x + sn * s.y, cs * s.y - sn * s.x, s.z};
     }
 
-    // The locally measured direction in the Boyer-Lindquist frame, from the
-    // coordinate velocities (kerr_direction's convention).
-    GARG_DEV f3 direction_bl() const {
-        const KsState v = ks_derivative(k, s);
-        const float r2 = r * r, q = r2 + k.a2;
+    // The locally measured direction in the Boyer-Lindquist frame (the
+    // convention of kerr_direction) from the coordinate velocities at the
+    // current point, whose Boyer-Lindquist position is `p`.
+    GARG_DEV f3 direction_bl(f3 p) const {
+        const KsState& v = e.d;
+        const float r = e.r, r2 = r * r, q = e.q;
         const float d = r2 * r2 + k.a2 * s.z * s.z;
-        const float rdot = (s.x * v.x + s.y * v.y) * r2 * r / d + s.z * v.z * r * q / d;
-        const float ct = fminf(fmaxf(s.z / r, -1.0f), 1.0f);
+        const float rdot = ((s.x * v.x + s.y * v.y) * r2 * r + s.z * v.z * r * q) / d;
+        const float ct = fminf(fmaxf(s.z * e.ir, -1.0f), 1.0f);
         const float st = sqrtf(fmaxf(1.0f - ct * ct, 1e-12f));
         const float rho2 = fmaxf(s.x * s.x + s.y * s.y, 1e-30f);
+        const float irho = __builtin_amdgcn_rsqf(rho2);
         // theta' from z = r cos(theta) away from the axis, from
         // rho = sqrt(r^2 + a^2) sin(theta) near it, where the first divides
         // two small numbers.
@@ -1042,14 +1079,12 @@ struct KerrSchildMarch {
             thdot = (ct * rdot - v.z) / (r * st);
         } else {
             const float rq = sqrtf(q);
-            const float rhodot = (s.x * v.x + s.y * v.y) * __builtin_amdgcn_rsqf(rho2);
+            const float rhodot = (s.x * v.x + s.y * v.y) * irho;
             thdot = (rhodot - r * rdot * st / rq) / (rq * ct);
         }
         const float delta = r2 - 2.0f * r + k.a2;
         const float phdot = (s.x * v.y - s.y * v.x) / rho2 + k.a * rdot / q - k.a * rdot / delta;
-        const float phi = atan2f(s.y, s.x) - atan2f(k.a, r) - (ks_phi_shift(k, r) - k.phase);
-        float sp, cp;
-        fast_sincos(phi, &sp, &cp);
+        const float cp = p.x * irho, sp = p.y * irho;
         const f3 r_hat{st * cp, st * sp, ct};
         const f3 theta_hat{ct * cp, ct * sp, -st};
         const f3 phi_hat{-sp, cp, 0.0f};
@@ -1059,82 +1094,103 @@ struct KerrSchildMarch {
     }
 
     GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget) {
-        const garg_geodesic_settings& gs = u.geodesic;
-        const bool has_volume = gather && sc.volume.voxels != nullptr;
+        run_loop<false>(kerr_loop(u, sc, gather, compensated), budget);
+    }
+
+    // Advance by at most `budget` steps (the contract of KerrFastMarch).
+    template <bool UNIFORM, class Tick = NoTick>
+    GARG_DEV void run_loop(KerrLoop l, uint32_t budget, Tick tick = Tick{}) {
+        if constexpr (UNIFORM) l = wave_uniform(l);
+        const bool has_volume = l.gather != 0u;
+        const bool compensated = l.compensated != 0u;
         for (uint32_t n = 0; n < budget && !done; ++n) {
-            if (steps >= gs.step_cap) {
+            tick(n);
+            if (steps >= l.step_cap) {
                 done = true;
                 break;
             }
+            const float r = e.r;
             // The Boyer-Lindquist step policy without its pole factor.
-            const float h = fminf(fmaxf(fminf(gs.step_factor * r, 0.2f * (r - k.rplus)), GARG_F(KERR_STEP_FLOOR)),
-                                  gs.step_max);
+            const float h = fminf(fmaxf(fminf(l.step_factor * r, 0.2f * (r - k.rplus)), GARG_F(KERR_STEP_FLOOR)),
+                                  l.step_max);
             const float hh = 0.5f * h;
-            const KsState k1 = ks_derivative(k, s);
-            const KsState k2 = ks_derivative(k, ks_axpy(s, k1, hh));
-            const KsState k3 = ks_derivative(k, ks_axpy(s, k2, hh));
-            const KsState k4 = ks_derivative(k, ks_axpy(s, k3, h));
+            const KsState k2 = ks_eval(k, ks_axpy(s, e.d, hh)).d;
+            const KsState k3 = ks_eval(k, ks_axpy(s, k2, hh)).d;
+            const KsState k4 = ks_eval(k, ks_axpy(s, k3, h)).d;
             const float w = h * (1.0f / 6.0f);
-            two_sum(s.x, c.x, ((k1.x + k2.x * 2.0f) + (k3.x * 2.0f + k4.x)) * w);
-            two_sum(s.y, c.y, ((k1.y + k2.y * 2.0f) + (k3.y * 2.0f + k4.y)) * w);
-            two_sum(s.z, c.z, ((k1.z + k2.z * 2.0f) + (k3.z * 2.0f + k4.z)) * w);
-            two_sum(s.px, c.px, ((k1.px + k2.px * 2.0f) + (k3.px * 2.0f + k4.px)) * w);
-            two_sum(s.py, c.py, ((k1.py + k2.py * 2.0f) + (k3.py * 2.0f + k4.py)) * w);
-            two_sum(s.pz, c.pz, ((k1.pz + k2.pz * 2.0f) + (k3.pz * 2.0f + k4.pz)) * w);
+            two_sum(s.x, c.x, ((e.d.x + k2.x * 2.0f) + (k3.x * 2.0f + k4.x)) * w);
+            two_sum(s.y, c.y, ((e.d.y + k2.y * 2.0f) + (k3.y * 2.0f + k4.y)) * w);
+            two_sum(s.z, c.z, ((e.d.z + k2.z * 2.0f) + (k3.z * 2.0f + k4.z)) * w);
+            two_sum(s.px, c.px, ((e.d.px + k2.px * 2.0f) + (k3.px * 2.0f + k4.px)) * w);
+            two_sum(s.py, c.py, ((e.d.py + k2.py * 2.0f) + (k3.py * 2.0f + k4.py)) * w);
+            two_sum(s.pz, c.pz, ((e.d.pz + k2.pz * 2.0f) + (k3.pz * 2.0f + k4.pz)) * w);
             if (!compensated) c = KsState{0, 0, 0, 0, 0, 0};
             steps += 1;
-            r = ks_radius(k.a2, s.x, s.y, s.z);
+            e = ks_eval(k, s);
+            const float r1 = e.r;
             // Drift gauges: the null residual 2 Sigma H everywhere, the Carter
-            // constant away from the axis (its formula divides by sin^2),
-            // the axial angular momentum everywhere.
+            // constant away from the axis (its formula divides by
+            // sin^2(theta)), the axial angular momentum everywhere.
             {
-                const float q = r * r + k.a2, d = r * r * r * r + k.a2 * s.z * s.z;
-                const float f = 2.0f * r * r * r / d;
-                const float lx = (r * s.x + k.a * s.y) / q, ly = (r * s.y - k.a * s.x) / q, lz = s.z / r;
-                const float big = k.E + ((lx * s.px + ly * s.py) + lz * s.pz);
                 const float hamilton =
-                    0.5f * ((((s.px * s.px + s.py * s.py) + s.pz * s.pz) - k.E * k.E) - f * big * big);
-                const float sigma = r * r + k.a2 * (s.z * s.z) / (r * r);
-                float dn = fabsf(2.0f * sigma * hamilton) * inv_scale / fmaxf(r * r, 1.0f);
-                if (s.x * s.x + s.y * s.y > 1e-4f * (r * r + k.a2))
-                    dn = fmaxf(dn, fabsf(carter_of(s, r) - carter0) * inv_scale);
+                    0.5f * ((((s.px * s.px + s.py * s.py) + s.pz * s.pz) - k.E * k.E) - e.f * e.big * e.big);
+                const float ct = s.z * e.ir;
+                const float sigma = r1 * r1 + k.a2 * (ct * ct);
+                float dn = fabsf(2.0f * sigma * hamilton) * inv_scale * __builtin_amdgcn_rcpf(fmaxf(r1 * r1, 1.0f));
+                if (s.x * s.x + s.y * s.y > 1e-4f * e.q) dn = fmaxf(dn, fabsf(carter() - carter0) * inv_scale);
                 dn = fmaxf(dn, fabsf((s.x * s.py - s.y * s.px) - l0) * inv_root_scale);
                 drift = fmaxf(drift, dn);
             }
-            const f3 p = position_bl();
-            const f3 dp = p - prev;
-            const float path = __builtin_amdgcn_sqrtf(dot(dp, dp));
-            g.travelled += path;
-            if (r <= capture) {
-                outcome = GARG_RAY_CAPTURED;
-                done = true;
-                break;
+            // The chord between the two Boyer-Lindquist points: with
+            // delta = psi(r1) - psi(r0),
+            // |P1 - P0|^2 = |p1 - p0|^2 + 2 (1 - cos delta) (p0.p1)_xy
+            //               - 2 sin(delta) (p0 x p1)_z.
+            // atan(a/r1) - atan(a/r0) = atan(a (r0 - r1) / (r0 r1 + a^2)) and
+            // the log term's difference is ln(1 + (r1 - r0)(r+ - r-)
+            // / ((r1 - r-)(r0 - r+))).
+            {
+                const float r0 = prev_r, dr = r1 - r0;
+                const float d_atan = small_atan(-k.a * dr * __builtin_amdgcn_rcpf(r0 * r1 + k.a2));
+                const float d_log = small_log1p(dr * (k.rp - k.rm) * __builtin_amdgcn_rcpf((r1 - k.rm) * (r0 - k.rp)));
+                const float delta = d_atan + k.shift_scale * d_log;
+                const float dx = s.x - prev.x, dy = s.y - prev.y, dz = s.z - prev.z;
+                const float dot_xy = prev.x * s.x + prev.y * s.y;
+                const float cross_z = prev.x * s.y - prev.y * s.x;
+                const float chord2 = ((dx * dx + dy * dy) + dz * dz) + 2.0f * versine(delta) * dot_xy -
+                                     2.0f * small_sin(delta) * cross_z;
+                const float path = __builtin_amdgcn_sqrtf(fmaxf(chord2, 0.0f));
+                g.travelled += path;
+                if (r1 <= capture) {
+                    outcome = GARG_RAY_CAPTURED;
+                    done = true;
+                    break;
+                }
+                if (has_volume && fabsf(s.z) < GARG_F(DISK_SLAB_HALF_HEIGHT)) {
+                    const f3 p = position_bl();
+                    const float fbl = 1.0f - 2.0f * r1 * __builtin_amdgcn_rcpf(r1 * r1 + k.a2 * (s.z * e.ir) * (s.z * e.ir));
+                    g.sample(l.volume, l.opacity, l.redshift != 0u, p, path, fmaxf(fbl, 0.0f),
+                             [&]() { return direction_bl(p); });
+                }
             }
-            if (has_volume) {
-                const float sigma = r * r + k.a2 * (s.z * s.z) / (r * r);
-                const float f = 1.0f - 2.0f * r / sigma;
-                g.sample(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, fmaxf(f, 0.0f),
-                         [&]() { return direction_bl(); });
-            }
-            prev = p;
-            if (r >= gs.escape_radius) {
+            prev = f3{s.x, s.y, s.z};
+            prev_r = r1;
+            if (r1 >= l.escape_radius) {
                 outcome = GARG_RAY_ESCAPED;
                 done = true;
                 break;
             }
         }
-        if (!done && steps >= gs.step_cap) done = true;
+        if (!done && steps >= l.step_cap) done = true;
     }
 
-    // Whether the ray falls into the hole (outward speed negative when the
-    // budget ran out, as the Boyer-Lindquist form reads p_r).
+    // Whether the ray falls into the hole: outward speed negative when the
+    // budget ran out, as the Boyer-Lindquist form reads p_r.
     GARG_DEV bool falls_in() const {
         if (outcome == GARG_RAY_CAPTURED) return true;
         if (outcome != GARG_RAY_EXHAUSTED) return false;
-        const KsState v = ks_derivative(k, s);
-        return (s.x * v.x + s.y * v.y) * r * r + s.z * v.z * (r * r + k.a2) < 0.0f;
+        return (s.x * e.d.x + s.y * e.d.y) * e.r * e.r + s.z * e.d.z * e.q < 0.0f;
     }
-    GARG_DEV f3 final_direction() const { return direction_bl(); }
+    GARG_DEV f3 final_direction() const { return direction_bl(position_bl()); }
     GARG_DEV f3 final_position() const { return position_bl(); }
 };
 

@@ -105,4 +105,53 @@ GARG_HD float versine(float d) {
     return 1.0f - c;
 }
 
+// atan x for the small arguments of the Kerr-Schild frame rotation: its
+// Taylor series to x^9 for |x| <= 1/8 (the first omitted term, x^11 / 11,
+// is below 1e-10 of the result), otherwise atanf.
+GARG_HD float small_atan(float x) {
+    if (fabsf(x) <= 0.125f) {
+        float x2 = x * x;
+        float p = fmaf(x2, 1.11111111111111111e-01f, -1.42857142857142857e-01f);
+        p = fmaf(x2, p, 2.0e-01f);
+        p = fmaf(x2, p, -3.33333333333333333e-01f);
+        p = p * x2;
+        return fmaf(x, p, x);
+    }
+    return atanf(x);
+}
+
+// ln |1 + e| for the small ratios of the Kerr-Schild frame rotation: its
+// series to e^8 for |e| <= 1/16 (the first omitted term, e^9 / 9, is below
+// 1e-10 of the result), otherwise log1pf (logf of |1 + e| where 1 + e
+// falls below 1/2).
+GARG_HD float small_log1p(float e) {
+    if (fabsf(e) <= 0.0625f) {
+        float p = fmaf(e, -0.125f, 1.42857142857142857e-01f);
+        p = fmaf(e, p, -1.66666666666666667e-01f);
+        p = fmaf(e, p, 2.0e-01f);
+        p = fmaf(e, p, -0.25f);
+        p = fmaf(e, p, 3.33333333333333333e-01f);
+        p = fmaf(e, p, -0.5f);
+        p = p * e;
+        return fmaf(e, p, e);
+    }
+    return e > -0.5f ? log1pf(e) : logf(fmaxf(fabsf(1.0f + e), 1e-30f));
+}
+
+// sin d: its Taylor series to d^9 for |d| <= 1/4 (the first omitted term is
+// below 4e-13 of the result), otherwise from fast_sincos.
+GARG_HD float small_sin(float d) {
+    if (fabsf(d) <= GARG_SHIFT_LIMIT) {
+        float d2 = d * d;
+        float p = fmaf(d2, 2.75573192239858907e-06f, -1.98412698412698413e-04f);
+        p = fmaf(d2, p, 8.33333333333333333e-03f);
+        p = fmaf(d2, p, -1.66666666666666667e-01f);
+        p = p * d2;
+        return fmaf(d, p, d);
+    }
+    float s, c;
+    fast_sincos(d, &s, &c);
+    return s;
+}
+
 }  // namespace garg

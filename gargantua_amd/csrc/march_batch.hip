@@ -105,12 +105,15 @@ GARG_DEV void age_priority(bool live, uint32_t started) {
     else __builtin_amdgcn_s_setprio(0);
 }
 
-template <bool KERR>
+// The ray state machine a lane runs: Schwarzschild (with the jump), Kerr in
+// Boyer-Lindquist coordinates (the FAST form) or Kerr in Kerr-Schild
+// coordinates.
+template <int KIND>
 struct Lane;
 
-template <>
-struct Lane<true> {
-    KerrFastMarch m;
+template <class March>
+struct KerrLane {
+    March m;
     GARG_DEV bool start(const BatchRequest& r, uint32_t px, uint32_t py, CounterTable&, uint32_t, uint32_t) {
         const garg_march_uniforms& u = r.uniforms;
         f3 dir = camera_direction(u, px, py);
@@ -130,7 +133,13 @@ struct Lane<true> {
 };
 
 template <>
-struct Lane<false> {
+struct Lane<BATCH_KERR> : KerrLane<KerrFastMarch> {};
+
+template <>
+struct Lane<BATCH_KERR_SCHILD> : KerrLane<KerrSchildMarch> {};
+
+template <>
+struct Lane<BATCH_SCHWARZSCHILD> {
     SchwMarchT<true> m;
     // Rays that never enter the integration sphere finish at the jump.
     GARG_DEV bool start(const BatchRequest& r, uint32_t px, uint32_t py, CounterTable& t, uint32_t request,
@@ -219,8 +228,8 @@ GARG_DEV uint64_t claim_rays(const BatchArgs& a, uint32_t home, uint32_t& queue,
 }
 
 // Start the lane's ray g of the batch; false if it has nothing to march.
-template <bool KERR>
-GARG_DEV bool start_ray(const BatchArgs& a, Lane<KERR>& ray, LaneRay& id, CounterTable& table, uint64_t g) {
+template <int KIND>
+GARG_DEV bool start_ray(const BatchArgs& a, Lane<KIND>& ray, LaneRay& id, CounterTable& table, uint64_t g) {
     uint32_t r = find_request(a, g);
     const BatchRequest& req = a.requests[r];
     uint32_t px, py;
@@ -234,8 +243,8 @@ GARG_DEV bool start_ray(const BatchArgs& a, Lane<KERR>& ray, LaneRay& id, Counte
 // and scene are scalar loads held in SGPRs, and one copy of the loop serves
 // every mix of requests. Finished rays are stored; returns whether the lane
 // is still live.
-template <bool KERR>
-GARG_DEV bool march_lanes(const BatchArgs& a, Lane<KERR>& ray, const LaneRay& id, CounterTable& table, bool live,
+template <int KIND>
+GARG_DEV bool march_lanes(const BatchArgs& a, Lane<KIND>& ray, const LaneRay& id, CounterTable& table, bool live,
                           uint32_t budget) {
     bool pending = live;
     for (;;) {
@@ -258,8 +267,11 @@ GARG_DEV bool march_lanes(const BatchArgs& a, Lane<KERR>& ray, const LaneRay& id
 
 // Lane-granular refill: a wave keeps marching while more than
 // `refill_threshold` lanes are live, then tops its finished lanes up.
-template <bool KERR, int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void march_batch_kernel(BatchArgs a) {
+// Kerr-Schild rays carry more state than 128 registers hold next to the
+// refill bookkeeping, so that kind runs three waves per SIMD here.
+template <int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KIND == BATCH_KERR_SCHILD ? 3 : 4))) void
+march_batch_kernel(BatchArgs a) {
     __shared__ uint32_t s_counts[GARG_BATCH_MAX_REQUESTS * 4];
     __shared__ unsigned long long s_steps[GARG_BATCH_MAX_REQUESTS];
     clear_counters<BLOCK>(s_counts, s_steps, a.request_count);
@@ -270,7 +282,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     uint32_t queue = home;
     bool drained = false;
 
-    Lane<KERR> ray;
+    Lane<KIND> ray;
     bool live = false;
     LaneRay id{0u, 0u};
     uint32_t started = 0;
@@ -311,11 +323,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 // frame in the order the hardware would dispatch it and keeps the tail
 // short. No ray state outlives a tile, so the kernel needs no more
 // registers than the one-launch-per-frame march: five waves per SIMD for
-// Kerr (96 VGPRs), four for Schwarzschild, whose jump and gather need more
-// than 96 registers without spilling.
-template <bool KERR, int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KERR ? 5 : 4))) void march_tiles_kernel(
-    BatchArgs a) {
+// Boyer-Lindquist Kerr (96 VGPRs), four for Schwarzschild, whose jump and
+// gather need more than 96 registers without spilling, and for Kerr-Schild.
+template <int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KIND == BATCH_KERR ? 5 : 4))) void
+march_tiles_kernel(BatchArgs a) {
     __shared__ uint32_t s_counts[GARG_BATCH_MAX_REQUESTS * 4];
     __shared__ unsigned long long s_steps[GARG_BATCH_MAX_REQUESTS];
     clear_counters<BLOCK>(s_counts, s_steps, a.request_count);
@@ -328,7 +340,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KERR ? 5 
         // Request ranges are 64-aligned, so the whole tile is one request.
         const uint32_t r = find_request(a, base);
         const BatchRequest& req = a.requests[r];
-        Lane<KERR> ray;
+        Lane<KIND> ray;
         uint32_t px = 0, py = 0;
         bool live = lane < got && ray_pixel(req, base + lane, px, py);
         const uint32_t pixel = py * req.uniforms.resolution[0] + px;
@@ -367,47 +379,51 @@ __global__ void queue_init_kernel(unsigned long long* queue, uint64_t total) {
     queue[q * QUEUE_STRIDE + 1] = end;
 }
 
-template <bool KERR>
+template <int KIND>
 static hipError_t launch_block(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream) {
     const bool tiles = a.refill_threshold == 0u;
     switch (block) {
         case 64:
-            if (tiles) march_tiles_kernel<KERR, 64><<<grid, 64, 0, stream>>>(a);
-            else march_batch_kernel<KERR, 64><<<grid, 64, 0, stream>>>(a);
+            if (tiles) march_tiles_kernel<KIND, 64><<<grid, 64, 0, stream>>>(a);
+            else march_batch_kernel<KIND, 64><<<grid, 64, 0, stream>>>(a);
             break;
         case 128:
-            if (tiles) march_tiles_kernel<KERR, 128><<<grid, 128, 0, stream>>>(a);
-            else march_batch_kernel<KERR, 128><<<grid, 128, 0, stream>>>(a);
+            if (tiles) march_tiles_kernel<KIND, 128><<<grid, 128, 0, stream>>>(a);
+            else march_batch_kernel<KIND, 128><<<grid, 128, 0, stream>>>(a);
             break;
         case 256:
-            if (tiles) march_tiles_kernel<KERR, 256><<<grid, 256, 0, stream>>>(a);
-            else march_batch_kernel<KERR, 256><<<grid, 256, 0, stream>>>(a);
+            if (tiles) march_tiles_kernel<KIND, 256><<<grid, 256, 0, stream>>>(a);
+            else march_batch_kernel<KIND, 256><<<grid, 256, 0, stream>>>(a);
             break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+template <int KIND>
+static uint32_t resident_blocks(bool tiles, uint32_t block) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define GARG_OCC(B)                                                                                       \
+    e = tiles ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_tiles_kernel<KIND, B>, B, 0)      \
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_batch_kernel<KIND, B>, B, 0)
+    if (block == 64) GARG_OCC(64);
+    else if (block == 128) GARG_OCC(128);
+    else if (block == 256) GARG_OCC(256);
+#undef GARG_OCC
+    return e == hipSuccess && n > 0 ? (uint32_t)n : 0u;
+}
+
 // Workgroups of the batched march one compute unit holds at once: a
 // persistent grid beyond that only queues workgroups that find the rays
 // already taken.
-uint32_t march_batch_resident_blocks(bool kerr, bool tiles, uint32_t block) {
-    int n = 0;
-    hipError_t e = hipErrorInvalidValue;
-#define GARG_OCC(K, B)                                                                                    \
-    e = tiles ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_tiles_kernel<K, B>, B, 0)         \
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_batch_kernel<K, B>, B, 0)
-    if (kerr) {
-        if (block == 64) GARG_OCC(true, 64);
-        else if (block == 128) GARG_OCC(true, 128);
-        else if (block == 256) GARG_OCC(true, 256);
-    } else {
-        if (block == 64) GARG_OCC(false, 64);
-        else if (block == 128) GARG_OCC(false, 128);
-        else if (block == 256) GARG_OCC(false, 256);
+uint32_t march_batch_resident_blocks(uint32_t kind, bool tiles, uint32_t block) {
+    switch (kind) {
+        case BATCH_SCHWARZSCHILD: return resident_blocks<BATCH_SCHWARZSCHILD>(tiles, block);
+        case BATCH_KERR: return resident_blocks<BATCH_KERR>(tiles, block);
+        case BATCH_KERR_SCHILD: return resident_blocks<BATCH_KERR_SCHILD>(tiles, block);
+        default: return 0u;
     }
-#undef GARG_OCC
-    return e == hipSuccess && n > 0 ? (uint32_t)n : 0u;
 }
 
 hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream) {
@@ -415,7 +431,12 @@ hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block,
     else queue_init_kernel<<<1, 64, 0, stream>>>(a.queue, a.total_rays);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return a.requests_kerr ? launch_block<true>(a, grid, block, stream) : launch_block<false>(a, grid, block, stream);
+    switch (a.kind) {
+        case BATCH_SCHWARZSCHILD: return launch_block<BATCH_SCHWARZSCHILD>(a, grid, block, stream);
+        case BATCH_KERR: return launch_block<BATCH_KERR>(a, grid, block, stream);
+        case BATCH_KERR_SCHILD: return launch_block<BATCH_KERR_SCHILD>(a, grid, block, stream);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace garg

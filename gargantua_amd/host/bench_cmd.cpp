@@ -83,7 +83,7 @@ static void sweep_batch(Scene& s, const std::string& path) {
         for (uint32_t waves : {2u, 4u, 8u})
             for (uint32_t period : {8u, 16u, 32u, 64u, 128u})
                 for (uint32_t threshold : {16u, 32u, 48u, 63u}) {
-                    garg_batch_config c{block, waves, period, threshold};
+                    garg_batch_config c{block, waves, period, threshold, 0u};
                     Timing tm = time_pass(
                         s.ctx, [&] { check(garg_march_batch(s.ctx, nullptr, &r, 1, &c, nullptr), "marching"); }, 5);
                     char buf[128];

@@ -56,7 +56,8 @@ enum { GARG_GENERATOR_DISK = 0, GARG_GENERATOR_ANNULUS = 1 };
  * future work for its polar seam (docs/lab/07-kerr.md:33-45). It takes the
  * same launch, step policy (without the pole factor), gas and sky as the
  * Boyer-Lindquist forms; at spin 0 it is the FAST Schwarzschild form.
- * The batched march always runs the FAST form. */
+ * The batched march runs the FAST form, or for spinning requests the
+ * Kerr-Schild form when garg_batch_config.kerr_schild asks for it. */
 enum { GARG_INTEGRATOR_REFERENCE = 0, GARG_INTEGRATOR_FAST = 1, GARG_INTEGRATOR_KERR_SCHILD = 2 };
 
 typedef struct garg_ctx garg_ctx;
@@ -283,6 +284,8 @@ typedef struct {
     uint32_t refill_threshold;  /* refill a wave when at most this many of 64 lanes are
                                    live; 0 = whole-tile claims: a wave marches 64 rays to
                                    the end before it takes the next 64 */
+    uint32_t kerr_schild;       /* spinning requests: 0 = the Boyer-Lindquist FAST form,
+                                   1 = the Kerr-Schild form (regular on the spin axis) */
 } garg_batch_config;
 
 /* Rays of a request in tile order: 8x8 pixel tiles row-major, 64 rays each. */

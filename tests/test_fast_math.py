@@ -143,3 +143,46 @@ def test_versine_relative_accuracy():
     series, fallback = float(out[0]), float(out[1])
     print(f"versine relative error: series {series:.2e}, fallback {fallback:.2e}")
     assert series < 6e-7 and fallback < 2e-6
+
+
+SMALL = r"""
+#include <math.h>
+#include <stdio.h>
+#include "fast_math.hpp"
+int main(void) {
+    double worst_atan = 0, worst_log = 0, worst_sin = 0;
+    for (int i = -200000; i <= 200000; ++i) {
+        if (i == 0) continue;
+        float x = 0.2f * (float)i / 200000.0f;      /* inside and past each series limit */
+        worst_atan = fmax(worst_atan, fabs(garg::small_atan(x) - atan((double)x)) / fabs(atan((double)x)));
+        worst_log = fmax(worst_log, fabs(garg::small_log1p(x) - log1p((double)x)) / fabs(log1p((double)x)));
+        float d = 1.5f * x;
+        worst_sin = fmax(worst_sin, fabs(garg::small_sin(d) - sin((double)d)) / fabs(sin((double)d)));
+    }
+    /* tiny arguments keep relative accuracy */
+    for (int k = 1; k < 1000; ++k) {
+        float x = (float)k * 1e-9f;
+        worst_atan = fmax(worst_atan, fabs(garg::small_atan(x) - atan((double)x)) / atan((double)x));
+        worst_log = fmax(worst_log, fabs(garg::small_log1p(x) - log1p((double)x)) / log1p((double)x));
+        worst_sin = fmax(worst_sin, fabs(garg::small_sin(x) - sin((double)x)) / sin((double)x));
+    }
+    printf("%g %g %g\n", worst_atan, worst_log, worst_sin);
+    return 0;
+}
+"""
+
+
+def test_small_argument_series():
+    """small_atan, small_log1p and small_sin (the Kerr-Schild frame
+    rotation's step-to-step differences) within a few float roundings of
+    the double-precision functions, relative to the result, across and
+    past each series' limit and at tiny arguments."""
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "h.cpp"), os.path.join(d, "h")
+        open(src, "w").write(SMALL)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+                        os.path.join(ROOT, "gargantua_amd", "csrc"), src, "-o", exe, "-lm"], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    worst_atan, worst_log, worst_sin = (float(v) for v in out)
+    print(f"relative error: atan {worst_atan:.2e}, log1p {worst_log:.2e}, sin {worst_sin:.2e}")
+    assert worst_atan < 3e-7 and worst_log < 3e-7 and worst_sin < 3e-7
