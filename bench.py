@@ -252,7 +252,7 @@ def main():
                      "peak_nonpacked": FP32_PEAK_NONPACKED_TFLOPS,
                      "flops_per_step": FLOPS_PER_KERR_STEP,
                      "kernel": ("march_tiles_kernel" if cfg.refill_threshold == 0 else "march_batch_kernel")
-                     + f"<true,{cfg.block}>"},
+                     + f"<1, {cfg.block}>"},
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "compute_units": info.compute_units},
     }
     if not args.no_mixed:
