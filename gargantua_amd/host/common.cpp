@@ -33,7 +33,7 @@ void check(int status, const char* what) {
 
 const char* kUsage =
     "usage: gargantua [--preset small|default|max] [--particles N] [--volume 96|128|192|256] [--seed S] "
-    "[--strategy march|bake] [--fps-cap 30|60] [--no-upscale] [--spin A] [--camera az,el,dist] "
+    "[--strategy march|bake] [--fps-cap 30|60] [--no-upscale] [--spin A] [--kerr-schild] [--camera az,el,dist] "
     "[--still out.png --width W --height H [--sequence N]] | bench [--sweep] | validate | serve [--rate R] "
     "[--seconds T] [--deadline D] [--frames F] [--tick-rays N] [--policy srpt|fifo] [--verify N] [--report PATH] | version | constants";
 
@@ -135,6 +135,8 @@ std::string parse_options(const std::vector<std::string>& args, Options& o) {
             if (!(err = value(a, text)).empty()) return err;
             if (!parse_double(text, s) || s < 0 || s > 0.998) return "spin must be between 0 and 0.998.";
             o.spin = s;
+        } else if (a == "--kerr-schild") {
+            o.kerr_schild = true;
         } else if (a == "--camera") {
             if (!(err = value(a, text)).empty()) return err;
             double p[3];

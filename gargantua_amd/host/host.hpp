@@ -42,6 +42,9 @@ struct Options {
     std::optional<int> sequence;
     std::optional<double> cam_azimuth, cam_elevation, cam_distance;
     double spin = 0.0;
+    // Spinning rays in Kerr-Schild coordinates (regular on the spin axis)
+    // instead of the reference's Boyer-Lindquist march.
+    bool kerr_schild = false;
     // serve: arrival rate, duration, scale-down of the workload for a bounded run
     double serve_rate = 20.0;
     double serve_seconds = 300.0;

@@ -74,6 +74,16 @@ def test_still_writes_a_png(tmp_path):
 
 
 @pytest.mark.gpu
+def test_still_in_kerr_schild_form(tmp_path):
+    out = tmp_path / "still.png"
+    r = run("--preset", "small", "--spin", "0.9", "--kerr-schild", "--still", str(out), "--width", "320",
+            "--height", "180")
+    assert r.returncode == 0, r.stderr
+    img = read_png(out)
+    assert img.shape == (180, 320, 4) and img[..., :3].max() > 0
+
+
+@pytest.mark.gpu
 def test_validate_passes_every_judged_golden():
     r = run("--preset", "small", "validate")
     print(r.stdout)
@@ -82,17 +92,20 @@ def test_validate_passes_every_judged_golden():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tick_rays,policy", [(4 << 20, "srpt"), (50_000, "srpt"), (50_000, "fifo")])
-def test_serve_results_equal_alone_and_in_the_mix(tmp_path, tick_rays, policy):
+@pytest.mark.parametrize("tick_rays,policy,kerr_schild", [(4 << 20, "srpt", False), (50_000, "srpt", False),
+                                                     (50_000, "fifo", False), (50_000, "srpt", True)])
+def test_serve_results_equal_alone_and_in_the_mix(tmp_path, tick_rays, policy, kerr_schild):
     """Every request re-rendered alone equals its render in the mix; a small
     per-tick ray budget splits frames over many ticks and leaves requests
-    waiting for budget, which each policy hands out in its own order."""
+    waiting for budget, which each policy hands out in its own order; the
+    spinning requests also in the Kerr-Schild form."""
     report = tmp_path / "serve.json"
     r = run("serve", "--preset", "small", "--rate", "6", "--seconds", "3", "--frames", "3", "--verify", "1",
-            "--tick-rays", str(tick_rays), "--policy", policy, "--report", str(report))
+            "--tick-rays", str(tick_rays), "--policy", policy, "--report", str(report),
+            *(["--kerr-schild"] if kerr_schild else []))
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     rep = json.loads(report.read_text())
     assert rep["completed"] > 0 and rep["verified"] > 0 and rep["mismatched"] == 0
     assert rep["latency_p99_s"] >= rep["latency_p50_s"] > 0
-    assert rep["policy"] == policy
+    assert rep["policy"] == policy and rep["kerr_schild"] == kerr_schild
