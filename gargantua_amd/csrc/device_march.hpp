@@ -876,7 +876,7 @@ struct KsState {
 };
 
 struct KsConsts {
-    float a, a2, E, rplus;
+    float a, a2, E;
     float rp, rm, shift_scale;  // horizons and a / (r+ - r-)
     float phase;                // ln |(r - r+) / (r - r-)| at the launch radius
 };
@@ -1008,7 +1008,6 @@ struct KerrSchildMarch {
         const float root = sqrtf(fmaxf(1.0f - k.a2, 0.0f));
         k.rp = 1.0f + root;
         k.rm = 1.0f - root;
-        k.rplus = k.rp;
         k.shift_scale = root > 0.0f ? spin / (k.rp - k.rm) : 0.0f;
         k.phase = ks_log_ratio(k, launched.r);
         s = ks_from_bl(k, launched);
@@ -1019,7 +1018,7 @@ struct KerrSchildMarch {
         const float scale = fmaxf(fabsf(carter0), k.E * k.E);
         inv_scale = 1.0f / scale;
         inv_root_scale = 1.0f / sqrtf(scale);
-        capture = k.rplus + GARG_F(KERR_CAPTURE_MARGIN);
+        capture = k.rp + GARG_F(KERR_CAPTURE_MARGIN);
         g.reset();
         prev = f3{s.x, s.y, s.z};
         prev_r = e.r;
@@ -1111,7 +1110,7 @@ struct KerrSchildMarch {
             }
             const float r = e.r;
             // The Boyer-Lindquist step policy without its pole factor.
-            const float h = fminf(fmaxf(fminf(l.step_factor * r, 0.2f * (r - k.rplus)), GARG_F(KERR_STEP_FLOOR)),
+            const float h = fminf(fmaxf(fminf(l.step_factor * r, 0.2f * (r - k.rp)), GARG_F(KERR_STEP_FLOOR)),
                                   l.step_max);
             const float hh = 0.5f * h;
             const KsState k2 = ks_eval(k, ks_axpy(s, e.d, hh)).d;
