@@ -897,6 +897,8 @@ int garg_batch_config_default(garg_ctx* ctx, garg_batch_config* out) {
 // Bands a request is split into in the batch queue (at most, and only while
 // the table has room for every band).
 static const uint32_t BATCH_BANDS = 4;
+// Largest Schwarzschild step budget marched without tile aging.
+static const uint32_t SCHWARZSCHILD_SHORT_BUDGET = 256;
 
 int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_request* requests, uint32_t count,
                      const garg_batch_config* config, garg_march_counters* total) {
@@ -935,7 +937,7 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
         BatchRequest* table = ctx->batch_requests + (size_t)part * GARG_BATCH_MAX_REQUESTS;
         // This part's requests and their ray ranges.
         uint32_t members[GARG_BATCH_MAX_REQUESTS], member_rays[GARG_BATCH_MAX_REQUESTS];
-        uint32_t m = 0;
+        uint32_t m = 0, max_cap = 0;
         for (uint32_t i = 0; i < count; ++i) {
             const garg_march_request& r = requests[i];
             if ((r.uniforms.spin > 0.0f) != kerr) continue;
@@ -945,6 +947,7 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
             if (rays == 0) continue;
             members[m] = i;
             member_rays[m++] = rays;
+            max_cap = std::max(max_cap, r.uniforms.geodesic.step_cap);
         }
         // Each request is queued as up to BATCH_BANDS bands of whole tiles
         // (horizontal bands of its image), the middle bands first and the
@@ -995,7 +998,14 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
         a.totals = ctx->batch_totals;
         a.compaction_period = config->compaction_period;
         a.refill_threshold = config->refill_threshold;
-        uint32_t resident = march_batch_resident_blocks(a.kind, config->refill_threshold == 0u, config->block);
+        // Tile aging keeps long-lived persistent waves from starving (the
+        // SIMD's issue arbitration favours the same wave slots); waves of
+        // Schwarzschild batches within the interactive 256-step budget live
+        // too briefly for that, and without it the kernel fits five waves
+        // per SIMD.
+        a.aging = kerr || max_cap > SCHWARZSCHILD_SHORT_BUDGET ? 1u : 0u;
+        uint32_t resident =
+            march_batch_resident_blocks(a.kind, a.aging != 0u, config->refill_threshold == 0u, config->block);
         uint32_t launch_grid = resident ? std::min(grid, resident * ctx->info.compute_units) : grid;
         HIP_TRY(launch_march_batch(a, launch_grid, config->block, s), "launching the batched march");
     }

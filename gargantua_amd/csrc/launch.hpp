@@ -61,13 +61,16 @@ struct BatchArgs {
     garg_march_counters* totals;
     uint32_t compaction_period;
     uint32_t refill_threshold;
+    uint32_t aging;                 // whole-tile claims: raise a tile's issue priority as it ages
+                                    // (always for Kerr; for Schwarzschild when a budget exceeds
+                                    // the interactive 256 steps)
 };
 
 // The batched march's ray state machines.
 enum { BATCH_SCHWARZSCHILD = 0, BATCH_KERR = 1, BATCH_KERR_SCHILD = 2 };
 
 hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
-uint32_t march_batch_resident_blocks(uint32_t kind, bool tiles, uint32_t block);
+uint32_t march_batch_resident_blocks(uint32_t kind, bool aging, bool tiles, uint32_t block);
 
 // Particles, structure of arrays.
 struct ParticleView {

@@ -323,10 +323,13 @@ march_batch_kernel(BatchArgs a) {
 // frame in the order the hardware would dispatch it and keeps the tail
 // short. No ray state outlives a tile, so the kernel needs no more
 // registers than the one-launch-per-frame march: five waves per SIMD for
-// Boyer-Lindquist Kerr (96 VGPRs), four for Schwarzschild, whose jump and
-// gather need more than 96 registers without spilling, and for Kerr-Schild.
-template <int KIND, int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KIND == BATCH_KERR ? 5 : 4))) void
+// Boyer-Lindquist Kerr (96 VGPRs), four for Kerr-Schild. AGING = false drops
+// the tile's priority aging: Schwarzschild batches of short budgets, whose
+// waves live too briefly to starve, then fit five waves per SIMD, which the
+// jump and gather do not next to the aging state.
+template <int KIND, int BLOCK, bool AGING>
+__global__ __launch_bounds__(BLOCK) __attribute__((
+    amdgpu_waves_per_eu(KIND == BATCH_KERR || (KIND == BATCH_SCHWARZSCHILD && !AGING) ? 5 : 4))) void
 march_tiles_kernel(BatchArgs a) {
     __shared__ uint32_t s_counts[GARG_BATCH_MAX_REQUESTS * 4];
     __shared__ unsigned long long s_steps[GARG_BATCH_MAX_REQUESTS];
@@ -352,7 +355,10 @@ march_tiles_kernel(BatchArgs a) {
         // ages in real time, so the oldest tiles finish first, as they do
         // when the hardware launches a fresh wave per tile.
         __builtin_amdgcn_s_setprio(0);
-        if (live) ray.template run<true>(req, 0xffffffffu, TileAging{__builtin_amdgcn_s_memrealtime()});
+        if (live) {
+            if constexpr (AGING) ray.template run<true>(req, 0xffffffffu, TileAging{__builtin_amdgcn_s_memrealtime()});
+            else ray.template run<true>(req, 0xffffffffu);
+        }
         if (live) ray.finish(req, table, r, pixel);
         // Claiming only now, not before the march, keeps the claim's result
         // out of the registers the march needs.
@@ -379,20 +385,20 @@ __global__ void queue_init_kernel(unsigned long long* queue, uint64_t total) {
     queue[q * QUEUE_STRIDE + 1] = end;
 }
 
-template <int KIND>
+template <int KIND, bool AGING>
 static hipError_t launch_block(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream) {
     const bool tiles = a.refill_threshold == 0u;
     switch (block) {
         case 64:
-            if (tiles) march_tiles_kernel<KIND, 64><<<grid, 64, 0, stream>>>(a);
+            if (tiles) march_tiles_kernel<KIND, 64, AGING><<<grid, 64, 0, stream>>>(a);
             else march_batch_kernel<KIND, 64><<<grid, 64, 0, stream>>>(a);
             break;
         case 128:
-            if (tiles) march_tiles_kernel<KIND, 128><<<grid, 128, 0, stream>>>(a);
+            if (tiles) march_tiles_kernel<KIND, 128, AGING><<<grid, 128, 0, stream>>>(a);
             else march_batch_kernel<KIND, 128><<<grid, 128, 0, stream>>>(a);
             break;
         case 256:
-            if (tiles) march_tiles_kernel<KIND, 256><<<grid, 256, 0, stream>>>(a);
+            if (tiles) march_tiles_kernel<KIND, 256, AGING><<<grid, 256, 0, stream>>>(a);
             else march_batch_kernel<KIND, 256><<<grid, 256, 0, stream>>>(a);
             break;
         default: return hipErrorInvalidValue;
@@ -400,12 +406,12 @@ static hipError_t launch_block(const BatchArgs& a, uint32_t grid, uint32_t block
     return hipGetLastError();
 }
 
-template <int KIND>
+template <int KIND, bool AGING>
 static uint32_t resident_blocks(bool tiles, uint32_t block) {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
-#define GARG_OCC(B)                                                                                       \
-    e = tiles ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_tiles_kernel<KIND, B>, B, 0)      \
+#define GARG_OCC(B)                                                                                          \
+    e = tiles ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_tiles_kernel<KIND, B, AGING>, B, 0) \
               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_batch_kernel<KIND, B>, B, 0)
     if (block == 64) GARG_OCC(64);
     else if (block == 128) GARG_OCC(128);
@@ -417,11 +423,13 @@ static uint32_t resident_blocks(bool tiles, uint32_t block) {
 // Workgroups of the batched march one compute unit holds at once: a
 // persistent grid beyond that only queues workgroups that find the rays
 // already taken.
-uint32_t march_batch_resident_blocks(uint32_t kind, bool tiles, uint32_t block) {
+uint32_t march_batch_resident_blocks(uint32_t kind, bool aging, bool tiles, uint32_t block) {
     switch (kind) {
-        case BATCH_SCHWARZSCHILD: return resident_blocks<BATCH_SCHWARZSCHILD>(tiles, block);
-        case BATCH_KERR: return resident_blocks<BATCH_KERR>(tiles, block);
-        case BATCH_KERR_SCHILD: return resident_blocks<BATCH_KERR_SCHILD>(tiles, block);
+        case BATCH_SCHWARZSCHILD:
+            return aging ? resident_blocks<BATCH_SCHWARZSCHILD, true>(tiles, block)
+                         : resident_blocks<BATCH_SCHWARZSCHILD, false>(tiles, block);
+        case BATCH_KERR: return resident_blocks<BATCH_KERR, true>(tiles, block);
+        case BATCH_KERR_SCHILD: return resident_blocks<BATCH_KERR_SCHILD, true>(tiles, block);
         default: return 0u;
     }
 }
@@ -432,9 +440,11 @@ hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     switch (a.kind) {
-        case BATCH_SCHWARZSCHILD: return launch_block<BATCH_SCHWARZSCHILD>(a, grid, block, stream);
-        case BATCH_KERR: return launch_block<BATCH_KERR>(a, grid, block, stream);
-        case BATCH_KERR_SCHILD: return launch_block<BATCH_KERR_SCHILD>(a, grid, block, stream);
+        case BATCH_SCHWARZSCHILD:
+            return a.aging ? launch_block<BATCH_SCHWARZSCHILD, true>(a, grid, block, stream)
+                           : launch_block<BATCH_SCHWARZSCHILD, false>(a, grid, block, stream);
+        case BATCH_KERR: return launch_block<BATCH_KERR, true>(a, grid, block, stream);
+        case BATCH_KERR_SCHILD: return launch_block<BATCH_KERR_SCHILD, true>(a, grid, block, stream);
         default: return hipErrorInvalidValue;
     }
 }

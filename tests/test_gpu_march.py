@@ -125,11 +125,13 @@ def test_batched_matches_plain_bitwise_kerr(ctx, threshold):
     print(f"rays {pc.rays} steps {pc.steps} captured {pc.captured}")
 
 
-@pytest.mark.parametrize("threshold", [32, 0])
-def test_batched_matches_plain_bitwise_schwarzschild_with_jump(ctx, threshold):
+@pytest.mark.parametrize("threshold,cap", [(32, 4096), (0, 4096), (0, 256)])
+def test_batched_matches_plain_bitwise_schwarzschild_with_jump(ctx, threshold, cap):
+    """Lane refill and whole-tile claims; whole tiles with tile aging (a
+    4096-step budget) and without it (the 256-step budget)."""
     vol = disk_volume(ctx)
     tables = api.Tables(ctx)
-    req = schw_request(ctx, 192, 108, vol, tables)
+    req = schw_request(ctx, 192, 108, vol, tables, cap)
     (plain, pc) = render_plain(ctx, req)
     ctx.march_batch([req], ctx.batch_config(compaction_period=16, refill_threshold=threshold))
     assert same_bits(plain, req["targets"].read())
