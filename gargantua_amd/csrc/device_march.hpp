@@ -89,6 +89,18 @@ GARG_DEV void two_sum(float& s, float& carry, float increment) {
     s = t;
 }
 
+// Dekker's Fast2Sum of s and y = sum * w + carry (one rounding): t + err
+// equals s + y exactly when |s| >= |y|, which holds for r, phi and theta
+// away from the pole and for p_r and p_theta away from their turning
+// points; in the few steps where it does not, t is still the rounded sum
+// and only that step's carry is approximate.
+GARG_DEV void fast_two_sum(float& s, float& carry, float sum, float w) {
+    float y = fmaf(sum, w, carry);
+    float t = s + y;
+    carry = y - (t - s);
+    s = t;
+}
+
 GARG_DEV void ray_apply(RayState& s, RayState& carry, const RayState& inc, bool compensated) {
     if (compensated) {
         two_sum(s.r, carry.r, inc.r);
@@ -793,18 +805,13 @@ struct KerrFastMarch {
             KerrEval k3 = kerr_eval_stage(k, e, r + k2.dr * hh, th, k2.dth * hh, pr + k2.dpr * hh, pth + k2.dpth * hh);
             KerrEval k4 = kerr_eval_stage(k, e, r + k3.dr * h, th, k3.dth * h, pr + k3.dpr * h, pth + k3.dpth * h);
             float w = h * (1.0f / 6.0f);
-            float ir = ((e.dr + k2.dr * 2.0f) + (k3.dr * 2.0f + k4.dr)) * w;
-            float ith = ((e.dth + k2.dth * 2.0f) + (k3.dth * 2.0f + k4.dth)) * w;
-            float iph = ((e.dph + k2.dph * 2.0f) + (k3.dph * 2.0f + k4.dph)) * w;
-            float ipr = ((e.dpr + k2.dpr * 2.0f) + (k3.dpr * 2.0f + k4.dpr)) * w;
-            float ipth = ((e.dpth + k2.dpth * 2.0f) + (k3.dpth * 2.0f + k4.dpth)) * w;
-            // TwoSum always; without compensation the carries stay zero, which
-            // makes each update the plain s + increment.
-            two_sum(r, cr, ir);
-            two_sum(th, cth, ith);
-            two_sum(ph, cph, iph);
-            two_sum(pr, cpr, ipr);
-            two_sum(pth, cpth, ipth);
+            // Compensated always; without compensation the carries stay zero,
+            // which makes each update the plain s + increment.
+            fast_two_sum(r, cr, (e.dr + k2.dr * 2.0f) + (k3.dr * 2.0f + k4.dr), w);
+            fast_two_sum(th, cth, (e.dth + k2.dth * 2.0f) + (k3.dth * 2.0f + k4.dth), w);
+            fast_two_sum(ph, cph, (e.dph + k2.dph * 2.0f) + (k3.dph * 2.0f + k4.dph), w);
+            fast_two_sum(pr, cpr, (e.dpr + k2.dpr * 2.0f) + (k3.dpr * 2.0f + k4.dpr), w);
+            fast_two_sum(pth, cpth, (e.dpth + k2.dpth * 2.0f) + (k3.dpth * 2.0f + k4.dpth), w);
             if (!compensated) cr = cth = cph = cpr = cpth = 0.0f;
             steps += 1;
             e = kerr_eval(k, r, th, pr, pth);
