@@ -34,7 +34,7 @@ FP32_PEAK_NONPACKED_TFLOPS = 78.6
 # HBM bytes per launch of the batched Kerr march at this frame, from two
 # rocprofv3 counter runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a
 # 256 MiB device copy reads); profiles/march_kerr_1080p_a0.9_counters.md.
-TRAFFIC_BYTES_PER_LAUNCH = 489.0e6
+TRAFFIC_BYTES_PER_LAUNCH = 492.0e6
 
 
 def parse():
@@ -252,7 +252,7 @@ def main():
                      "peak_nonpacked": FP32_PEAK_NONPACKED_TFLOPS,
                      "flops_per_step": FLOPS_PER_KERR_STEP,
                      "kernel": ("march_tiles_kernel" if cfg.refill_threshold == 0 else "march_batch_kernel")
-                     + f"<1, {cfg.block}>"},
+                     + f"<1, {cfg.block}, true>"},
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "compute_units": info.compute_units},
     }
     if not args.no_mixed:
