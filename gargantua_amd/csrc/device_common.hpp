@@ -173,15 +173,44 @@ GARG_DEV f3 starfield(f3 d, uint32_t seed, const float4* blackbody) {
     return table_linear3(blackbody, (int)BLACKBODY_TABLE_SIZE, blackbody_coordinate(temperature)) * brightness;
 }
 
+// tanh x for x >= 0 in the fast march forms: its Taylor series to x^7 below
+// 1/8 (the first omitted term is 1.3e-9 of the result there), otherwise
+// 1 - 2 / (1 + e^(2x)) with the hardware exp2 and reciprocal (a few
+// roundings of the complement, about 1e-6 of tanh at x = 1/8 and less
+// above).
+GARG_DEV float fast_tanh(float x) {
+    if (x < 0.125f) {
+        float x2 = x * x;
+        float p = fmaf(x2, -5.3968253968253968e-02f, 1.3333333333333333e-01f);
+        p = fmaf(x2, p, -3.3333333333333333e-01f);
+        return fmaf(x * x2, p, x);
+    }
+    float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // e^(2x)
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+}
+
 // Redshift factor g = sqrt(f) / (gamma (1 + d . beta)) with the simulated
 // speed compressed below c and the tanh argument capped (Geodesic.h:225-236).
+// FAST evaluates it with fast_tanh, hardware square roots and reciprocals:
+// g = sqrt(f (1 - beta^2)) / (1 + d . beta).
+template <bool FAST = false>
 GARG_DEV float redshift_factor(float f, f3 direction, f3 gas) {
-    float magnitude = length(gas);
-    float speed = fminf(magnitude, 8.0f);
-    float beta = speed > 0.0f ? GARG_F(GAS_SPEED_CEILING) * tanhf(speed / GARG_F(GAS_SPEED_CEILING)) : 0.0f;
-    f3 velocity = magnitude > 0.0f ? gas * (beta / magnitude) : f3{0, 0, 0};
-    float gamma = 1.0f / sqrtf(1.0f - beta * beta);
-    return sqrtf(fmaxf(f, 0.0f)) / (gamma * (1.0f + dot(direction, velocity)));
+    if constexpr (FAST) {
+        float magnitude = __builtin_amdgcn_sqrtf(dot(gas, gas));
+        float speed = fminf(magnitude, 8.0f);
+        float beta = speed > 0.0f ? GARG_F(GAS_SPEED_CEILING) * fast_tanh(speed * (1.0f / GARG_F(GAS_SPEED_CEILING)))
+                                  : 0.0f;
+        f3 velocity = magnitude > 0.0f ? gas * (beta * __builtin_amdgcn_rcpf(magnitude)) : f3{0, 0, 0};
+        return __builtin_amdgcn_sqrtf(fmaxf(f, 0.0f) * (1.0f - beta * beta)) *
+               __builtin_amdgcn_rcpf(1.0f + dot(direction, velocity));
+    } else {
+        float magnitude = length(gas);
+        float speed = fminf(magnitude, 8.0f);
+        float beta = speed > 0.0f ? GARG_F(GAS_SPEED_CEILING) * tanhf(speed / GARG_F(GAS_SPEED_CEILING)) : 0.0f;
+        f3 velocity = magnitude > 0.0f ? gas * (beta / magnitude) : f3{0, 0, 0};
+        float gamma = 1.0f / sqrtf(1.0f - beta * beta);
+        return sqrtf(fmaxf(f, 0.0f)) / (gamma * (1.0f + dot(direction, velocity)));
+    }
 }
 
 // Front-to-back compositing state of one ray (march.metal:56-64, 81-102).
@@ -202,8 +231,10 @@ struct Gather {
 
     // One sample at `p` after a step of length `path`; `f` is the metric
     // factor for the gravitational redshift and `direction_of` yields the
-    // ray's locally measured direction when Doppler beaming needs it.
-    template <class DirectionOf>
+    // ray's locally measured direction when Doppler beaming needs it. FAST
+    // (the fast march forms) takes the redshift factor's fast evaluation
+    // and the hardware exp2 for the opacity.
+    template <bool FAST = false, class DirectionOf>
     GARG_DEV void sample(const VolumeView& vol, float opacity, bool redshift, f3 p, float path, float f,
                          DirectionOf direction_of) {
         if (!(fabsf(p.z) < GARG_F(DISK_SLAB_HALF_HEIGHT) && fabsf(p.x) < vol.half_xy && fabsf(p.y) < vol.half_xy &&
@@ -213,13 +244,14 @@ struct Gather {
         if (!(s.density > 0.0f)) return;
         float g3 = 1.0f;
         if (redshift) {
-            float g = redshift_factor(f, direction_of(), s.velocity);
-            float gravity = sqrtf(fmaxf(f, 0.0f));
+            float g = redshift_factor<FAST>(f, direction_of(), s.velocity);
+            float gravity = FAST ? __builtin_amdgcn_sqrtf(fmaxf(f, 0.0f)) : sqrtf(fmaxf(f, 0.0f));
             float bulk = fminf(s.density / GARG_F(BEAMING_DENSITY_FLOOR), 1.0f);
             float gr3 = gravity * gravity * gravity;
             g3 = gr3 + (g * g * g - gr3) * bulk;
         }
-        float alpha = 1.0f - expf(-opacity * s.density * path);
+        float alpha = FAST ? 1.0f - __builtin_amdgcn_exp2f(-opacity * s.density * path * 1.4426950408889634f)
+                           : 1.0f - expf(-opacity * s.density * path);
         f3 contribution = s.emission * (transmittance * g3 * path);
         color = color + contribution;
         float weight = contribution.x * 0.2126f + contribution.y * 0.7152f + contribution.z * 0.0722f;

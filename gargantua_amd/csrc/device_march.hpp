@@ -318,7 +318,7 @@ struct SchwMarchT {
             }
             if (has_volume) {
                 const PlaneRay& rr = ray;
-                g.sample(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, schw_f(ray.s.r),
+                g.sample<FAST>(sc.volume, u.opacity_scale, u.redshift != 0u, p, path, schw_f(ray.s.r),
                          [&]() { return FAST ? ray_direction_fast(rr) : ray_direction(rr); });
             }
             prev = p;
@@ -840,7 +840,7 @@ struct KerrFastMarch {
                 float f = 1.0f - 2.0f * r * e.inv_sigma;
                 const KerrEval& ee = e;
                 float rr = r;
-                g.sample(l.volume, l.opacity, l.redshift != 0u, p, path, fmaxf(f, 0.0f),
+                g.sample<true>(l.volume, l.opacity, l.redshift != 0u, p, path, fmaxf(f, 0.0f),
                          [&]() { return kerr_direction_sc(rr, ee.st, ee.ct, sp, cp, ee.dr, ee.dth, ee.dph); });
             }
             prev_rho = rho;
@@ -1174,7 +1174,7 @@ struct KerrSchildMarch {
                 if (has_volume && fabsf(s.z) < GARG_F(DISK_SLAB_HALF_HEIGHT)) {
                     const f3 p = position_bl();
                     const float fbl = 1.0f - 2.0f * r1 * __builtin_amdgcn_rcpf(r1 * r1 + k.a2 * (s.z * e.ir) * (s.z * e.ir));
-                    g.sample(l.volume, l.opacity, l.redshift != 0u, p, path, fmaxf(fbl, 0.0f),
+                    g.sample<true>(l.volume, l.opacity, l.redshift != 0u, p, path, fmaxf(fbl, 0.0f),
                              [&]() { return direction_bl(p); });
                 }
             }
