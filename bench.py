@@ -25,9 +25,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "RK4 geodesic steps/s at 1080p a=0.9 vs plain march; p50/p99 render latency"
-# Reference-form FP32 flops of one Kerr RK4 step of a live ray (4 Hamiltonian
-# evaluations, the RK4 update, step length, position, path length and the
-# Carter / null-residual drift gauges); the count is derived in DESIGN.md.
+# Reference-form FP32 flops of one Kerr RK4 step of a live ray, counted from
+# the formulas of Kerr.h:11-186 with +, -, *, /, sqrt, sin, cos = 1 and
+# FMA = 2: four Hamiltonian evaluations of 92 (368), the RK4 combination of
+# five components (68), the step length with its pole factor (10), the
+# Boyer-Lindquist to Cartesian position (13), the path length (10) and the
+# Carter and null-residual drift gauges (40).
 FLOPS_PER_KERR_STEP = 509
 FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak (packed)
 FP32_PEAK_NONPACKED_TFLOPS = 78.6
