@@ -265,6 +265,7 @@ def main():
         line["fp64"] = fp64_leg(ctx)
     if not args.no_mixed:
         line["kerr_640x360_x8"] = kerr_views_leg(ctx, api, vol, cfg)
+        line["kerr_presets"] = kerr_presets_leg(ctx, api, vol, cfg)
     if not args.no_schwarzschild:
         line["schwarzschild"] = schwarzschild_legs(ctx, api, vol, cfg, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
@@ -372,6 +373,36 @@ def kerr_views_leg(ctx, api, vol, cfg, views=8):
     out["speedup_vs_plain_fast"] = out["plain_fast_ms"] / out["batched_ms"]
     for r in reqs:
         r["targets"].close()
+    return out
+
+
+def kerr_presets_leg(ctx, api, vol, cfg):
+    """The reference's own Kerr operating points on the metric's scene
+    (a = 0.9, r_obs = 50, inclination 60 deg, 60 deg field of view, the
+    1M / 128^3 volume): the 960x540 interactive frame at the 256-step budget
+    (the reference reports "near 8 fps" for the whole interactive frame on
+    an M1 Pro, in its README.md:136-137) and the 1280x720 still at 4096 steps with
+    compensated accumulation ("under a second", docs/lab/07-kerr.md:49-50):
+    the batched march of one frame against the plain launch in each form."""
+    cam = api.orbit_camera(azimuth=0.0, elevation=math.radians(30.0), distance=50.0, fov_y_deg=60.0)
+    out = {}
+    for name, w, h, kind, cap in (("interactive_960x540_256", 960, 540, "interactive", 256),
+                                  ("still_1280x720_4096", 1280, 720, "still", 4096)):
+        u = api.uniforms(cam, w, h, api.settings(kind, step_cap=cap, escape_radius=60.0), spin=0.9,
+                         compensated=kind == "still", star_seed=1, volume_half_extent=30.0,
+                         drift_budget=1e-5 if kind == "still" else 1e-4)
+        t = api.Targets(ctx, w, h)
+        req = [{"uniforms": u, "targets": t, "volume": vol}]
+        t.reset_counters()
+        totals = ctx.march_batch(req, cfg)
+        ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, req, cfg), 1, 10)
+        leg = {"rays": int(totals.rays), "steps": int(totals.steps), "batched_ms": ms / 10}
+        for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
+            pms, _ = time_frames(ctx, api, lambda: ctx.march_image(u, t, vol, None, integrator), 1, 5)
+            leg[form + "_ms"] = pms / 5
+        leg["speedup_vs_plain_reference"] = leg["plain_reference_ms"] / leg["batched_ms"]
+        t.close()
+        out[name] = leg
     return out
 
 
