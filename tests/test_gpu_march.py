@@ -271,13 +271,16 @@ def settled_volume(ctx):
     p.close()
 
 
-def test_beaming_golden(ctx, settled_volume):
+@pytest.mark.parametrize("integrator", [api.INTEGRATOR_REFERENCE, api.INTEGRATOR_FAST], ids=["reference", "fast"])
+def test_beaming_golden(ctx, settled_volume, integrator):
+    """LensedRenderTests' beaming ratio against the FP64 render, in both forms
+    (the fast form evaluates the Doppler factor with its short tanh)."""
     g = load_golden("beaming")
     w, h = int(g["parameters"]["width"]), int(g["parameters"]["height"])
     cam = api.orbit_camera()
     u = api.uniforms(cam, w, h, api.settings("still"), stars=False, compensated=True)
     t = api.Targets(ctx, w, h)
-    ctx.march_image(u, t, settled_volume, None, api.INTEGRATOR_FAST)
+    ctx.march_image(u, t, settled_volume, None, integrator)
     color, _, _ = t.read()
     t.close()
     lum = np.ascontiguousarray((0.2126 * color[..., 0] + 0.7152 * color[..., 1] + 0.0722 * color[..., 2])
