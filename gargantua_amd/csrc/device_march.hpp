@@ -635,6 +635,7 @@ struct KerrEval {
     float null_residual;            // 2 Sigma H
     float carter;                   // Carter constant
     float inv_sigma;
+    float inv_r2;                   // 1 / max(r^2, 1) (GAUGES evaluations only)
 };
 
 GARG_DEV float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -663,7 +664,9 @@ GARG_DEV f3 kerr_direction_fast(float r, float st, float ct, float phi, float rd
     return kerr_direction_sc(r, st, ct, sp, cp, rdot, thdot, phdot);
 }
 
-// The right-hand sides at a state whose sin and cos of theta are given.
+// The right-hand sides at a state whose sin and cos of theta are given;
+// GAUGES also yields 1 / max(r^2, 1) for the null-residual gauge.
+template <bool GAUGES = false>
 GARG_DEV KerrEval kerr_eval_sc(const KerrConsts& k, float r, float st, float ct, float pr, float pth) {
     KerrEval e;
     e.st = st;
@@ -671,14 +674,21 @@ GARG_DEV KerrEval kerr_eval_sc(const KerrConsts& k, float r, float st, float ct,
     float r2 = r * r;
     float delta = (r2 - 2.0f * r) + k.a2;
     float sigma = r2 + k.a2 * (e.ct * e.ct);
-    float inv_sigma = fast_rcp(sigma);
-    float inv_delta = fast_rcp(delta);
-    // max(sin^2, 1e-8) = max(|sin|, 1e-4)^2: one reciprocal serves both
+    // max(sin^2, 1e-8) = max(|sin|, 1e-4)^2: one reciprocal of m serves both
     // clamped terms; max(sin, 1e-4) differs from max(|sin|, 1e-4) only past
-    // the pole, where sin < 0.
+    // the pole, where sin < 0. 1/Sigma, 1/Delta and 1/m (and for GAUGES
+    // 1/max(r^2, 1)) come from one reciprocal of their product: transcendental
+    // instructions issue at a fraction of the VALU rate.
     float m = fmaxf(fabsf(e.st), 1e-4f);
     float sin2 = m * m;
-    float inv_m = fast_rcp(m);
+    float sd = sigma * delta;
+    float M = GAUGES ? fmaxf(r2, 1.0f) : 1.0f;
+    float R = fast_rcp(sd * m * M);
+    float inv_m = GAUGES ? sd * M * R : sd * R;
+    float mR = GAUGES ? m * M * R : m * R;
+    float inv_sigma = delta * mR;
+    float inv_delta = sigma * mR;
+    e.inv_r2 = GAUGES ? sd * m * R : 1.0f;
     float inv_sin2 = inv_m * inv_m;
     float inv_ms = e.st >= 1e-4f ? inv_m : 1e4f;
     float P = (r2 + k.a2) * k.E - k.aL;
@@ -705,7 +715,7 @@ GARG_DEV KerrEval kerr_eval_sc(const KerrConsts& k, float r, float st, float ct,
 GARG_DEV KerrEval kerr_eval(const KerrConsts& k, float r, float th, float pr, float pth) {
     float st, ct;
     fast_sincos(th, &st, &ct);
-    return kerr_eval_sc(k, r, st, ct, pr, pth);
+    return kerr_eval_sc<true>(k, r, st, ct, pr, pth);
 }
 
 // A Runge-Kutta stage at theta + d from the evaluation `at` of theta: the
@@ -826,7 +836,7 @@ struct KerrFastMarch {
             float path = fast_sqrt((drho * drho + chord) + dz * dz);
             g.travelled += path;
             float carter_drift = fabsf(e.carter - carter0) * inv_scale;
-            float null_drift = fabsf(e.null_residual) * inv_scale * fast_rcp(fmaxf(r * r, 1.0f));
+            float null_drift = fabsf(e.null_residual) * inv_scale * e.inv_r2;
             drift = fmaxf(drift, fmaxf(carter_drift, null_drift));
             if (r <= capture) {
                 outcome = GARG_RAY_CAPTURED;
@@ -1044,9 +1054,12 @@ struct KerrSchildMarch {
     GARG_DEV float carter() const {
         const float rho2 = fmaxf(s.x * s.x + s.y * s.y, 1e-30f);
         const float ct = s.z * e.ir;
-        const float pth = (ct * e.q * (s.x * s.px + s.y * s.py) - e.r * rho2 * s.pz) * __builtin_amdgcn_rsqf(rho2 * e.q);
+        const float rs = __builtin_amdgcn_rsqf(rho2 * e.q);  // 1 / (rho sqrt(r^2 + a^2))
+        const float pth = (ct * e.q * (s.x * s.px + s.y * s.py) - e.r * rho2 * s.pz) * rs;
         const float L = s.x * s.py - s.y * s.px;
-        return pth * pth + ct * ct * (L * L * e.q * __builtin_amdgcn_rcpf(rho2) - k.a2 * k.E * k.E);
+        // L^2 (r^2 + a^2) / rho^2 = (L (r^2 + a^2) rs)^2
+        const float Lq = L * e.q * rs;
+        return pth * pth + ct * ct * (Lq * Lq - k.a2 * k.E * k.E);
     }
 
     // cos and sin of psi at the current point.
@@ -1142,7 +1155,7 @@ struct KerrSchildMarch {
                     0.5f * ((((s.px * s.px + s.py * s.py) + s.pz * s.pz) - k.E * k.E) - e.f * e.big * e.big);
                 const float ct = s.z * e.ir;
                 const float sigma = r1 * r1 + k.a2 * (ct * ct);
-                float dn = fabsf(2.0f * sigma * hamilton) * inv_scale * __builtin_amdgcn_rcpf(fmaxf(r1 * r1, 1.0f));
+                float dn = fabsf(2.0f * sigma * hamilton) * inv_scale * fminf(e.ir * e.ir, 1.0f);
                 if (s.x * s.x + s.y * s.y > 1e-4f * e.q) dn = fmaxf(dn, fabsf(carter() - carter0) * inv_scale);
                 dn = fmaxf(dn, fabsf((s.x * s.py - s.y * s.px) - l0) * inv_root_scale);
                 drift = fmaxf(drift, dn);
@@ -1156,8 +1169,10 @@ struct KerrSchildMarch {
             // / ((r1 - r-)(r0 - r+))).
             {
                 const float r0 = prev_r, dr = r1 - r0;
-                const float d_atan = small_atan(-k.a * dr * __builtin_amdgcn_rcpf(r0 * r1 + k.a2));
-                const float d_log = small_log1p(dr * (k.rp - k.rm) * __builtin_amdgcn_rcpf((r1 - k.rm) * (r0 - k.rp)));
+                const float u = r0 * r1 + k.a2, v = (r1 - k.rm) * (r0 - k.rp);
+                const float iuv = __builtin_amdgcn_rcpf(u * v);
+                const float d_atan = small_atan(-k.a * dr * (v * iuv));
+                const float d_log = small_log1p(dr * (k.rp - k.rm) * (u * iuv));
                 const float delta = d_atan + k.shift_scale * d_log;
                 const float dx = s.x - prev.x, dy = s.y - prev.y, dz = s.z - prev.z;
                 const float dot_xy = prev.x * s.x + prev.y * s.y;
