@@ -37,7 +37,7 @@ FP32_PEAK_NONPACKED_TFLOPS = 78.6
 # HBM bytes per launch of the batched Kerr march at this frame, from two
 # rocprofv3 counter runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a
 # 256 MiB device copy reads); profiles/march_kerr_1080p_a0.9_counters.md.
-TRAFFIC_BYTES_PER_LAUNCH = 488.0e6
+TRAFFIC_BYTES_PER_LAUNCH = 479.0e6
 
 
 def parse():
