@@ -202,6 +202,11 @@ int check_uniforms(const garg_march_uniforms* u) {
     float r = std::sqrt(u->camera_position[0] * u->camera_position[0] + u->camera_position[1] * u->camera_position[1] +
                         u->camera_position[2] * u->camera_position[2]);
     REQUIRE(std::isfinite(r) && r > 2.5f, "the camera must sit outside r = 2.5, not at r = %g.", (double)r);
+    if (u->spin > 0.0f) {
+        REQUIRE(u->geodesic.escape_radius <= (float)KERR_MAX_RADIUS && r <= (float)KERR_MAX_RADIUS,
+                "spinning rays are marched within r = %g: escape radius %g, camera at r = %g.", KERR_MAX_RADIUS,
+                (double)u->geodesic.escape_radius, (double)r);
+    }
     return GARG_OK;
 }
 
@@ -855,6 +860,17 @@ int garg_probe(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes,
     REQUIRE(uniforms->geodesic.step_cap <= (1u << 24), "the step cap is above 2^24.");
     REQUIRE(uniforms->spin > 0.0f || uniforms->sphere_radius <= 0.0f || tables,
             "the uniforms enable the sphere jump but no tables were given.");
+    if (uniforms->spin > 0.0f) {
+        REQUIRE(uniforms->geodesic.escape_radius <= (float)KERR_MAX_RADIUS,
+                "spinning rays are marched within r = %g, not to an escape radius of %g.", KERR_MAX_RADIUS,
+                (double)uniforms->geodesic.escape_radius);
+        for (uint32_t i = 0; i < count; ++i) {
+            const float* o = probes[i].origin;
+            float r = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+            REQUIRE(std::isfinite(r) && r <= (float)KERR_MAX_RADIUS,
+                    "spinning rays are marched within r = %g; probe %u starts at r = %g.", KERR_MAX_RADIUS, i, (double)r);
+        }
+    }
     uint64_t need = align256((uint64_t)count * sizeof(garg_ray_probe)) + (uint64_t)count * sizeof(garg_ray_probe_result);
     if (need > ctx->probe_scratch_bytes) {
         ctx_free(ctx, ctx->probe_scratch, ctx->probe_scratch_bytes);

@@ -254,7 +254,9 @@ int garg_targets_reset_counters(garg_targets* targets, garg_stream* stream);
 /* marchImage (march.metal:194-288): one thread per pixel, 8x8 groups, one
  * launch over the rectangle [tile_origin, tile_origin + extent) of a
  * targets image of size `resolution`. extent 0 means the whole image. The
- * counters of `targets` accumulate. */
+ * counters of `targets` accumulate. Spinning scenes keep the escape radius
+ * and the camera within r = KERR_MAX_RADIUS (GARG_E_ARGUMENT beyond), as do
+ * spinning probes their origins and every request of garg_march_batch. */
 int garg_march_image(garg_ctx* ctx, garg_stream* stream, const garg_march_uniforms* uniforms, garg_volume* volume,
                      garg_tables* tables, garg_targets* targets, uint32_t extent_w, uint32_t extent_h,
                      uint32_t integrator);

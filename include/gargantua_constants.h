@@ -26,6 +26,11 @@
 /* Scene radii (Constants.swift:88-121). */
 #define R_CAPTURE 2.02                      /* rays and particles end here */
 #define KERR_CAPTURE_MARGIN 0.05            /* Kerr rays end this far above r+ */
+/* Largest radius of a spinning ray (escape radius, camera, probe origins):
+ * the fast forms take one reciprocal of products growing like r^6, which
+ * stays far inside float range there. This build's own bound; the
+ * reference's scenes stay within r = 60. */
+#define KERR_MAX_RADIUS 1.0e5
 #define R_ESCAPE 60.0                       /* escaped beyond this radius */
 #define R_DISK_OUTER 24.0                   /* outer edge of seeded disk and annulus */
 #define R_FEED_INNER 20.0                   /* inner edge of the feeding annulus */

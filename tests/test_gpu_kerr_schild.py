@@ -194,3 +194,30 @@ def test_batched_kerr_schild_matches_plain_bitwise(ctx, threshold):
     assert same_bits(plain_a, a["targets"].read()) and same_bits(plain_b, b["targets"].read())
     for r in (a, b):
         r["targets"].close()
+
+
+def test_spinning_rays_beyond_the_radius_bound_are_refused(ctx):
+    """The fast Kerr forms take one reciprocal of products growing like r^6;
+    spinning rays are therefore marched within r = 1e5, and the entry points
+    refuse an escape radius, camera or probe origin beyond it with an
+    argument error rather than march garbage."""
+    import math
+    cam, g, u, grid = view(2, 2)
+    far = api.settings("still", escape_radius=2.0e5)
+    uf = api.uniforms(cam, 1, 1, far, drift_budget=1.0, redshift=False, stars=False, spin=SPIN, compensated=True)
+    with pytest.raises(api.GargError, match="within r = 100000"):
+        ctx.probe(grid, uf, integrator=api.INTEGRATOR_FAST)
+    origin, direction = grid[0]
+    with pytest.raises(api.GargError, match="probe 0 starts at r"):
+        ctx.probe([(np.asarray(origin) * 5e3, direction)], u, integrator=api.INTEGRATOR_KERR_SCHILD)
+    t = api.Targets(ctx, 16, 16)
+    um = api.uniforms(cam, 16, 16, far, spin=SPIN)
+    with pytest.raises(api.GargError, match="within r = 100000"):
+        ctx.march_image(um, t, None, None, api.INTEGRATOR_FAST)
+    with pytest.raises(api.GargError, match="within r = 100000"):
+        ctx.march_batch([{"uniforms": um, "targets": t}])
+    # Spin 0 has no such bound (the deflection golden launches at r = 50000).
+    u0 = api.uniforms(cam, 16, 16, far)
+    ctx.march_image(u0, t, None, None, api.INTEGRATOR_FAST)
+    assert math.isfinite(float(t.read()[1][..., 0].max()))
+    t.close()
