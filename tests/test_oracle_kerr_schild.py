@@ -148,3 +148,24 @@ def test_ks_converges_over_the_pole_where_boyer_lindquist_does_not():
         assert ks.outcome == fine.outcome == bl.outcome == 1
         assert angle(dks, dfine) < 1e-5 and ks.drift < 1e-7
         assert angle(dbl, dfine) > 10 * angle(dks, dfine) and bl.drift > 1e-5
+
+
+def test_ks_position_is_the_boyer_lindquist_position():
+    """orc_ks_position draws a Kerr-Schild state where the renderer draws
+    the same event in Boyer-Lindquist coordinates: sqrt(r^2 + a^2)
+    sin(theta) (cos phi, sin phi), r cos(theta) with phi the Boyer-Lindquist
+    azimuth (orc_kerr_position of orc_ks_to_bl), along integrated rays, not
+    only at the launch, where phi~ = phi."""
+    for a in (0.5, 0.9):
+        for u, v in ((0.2, 0.1), (-0.1, 0.25)):
+            ks = to_ks(a, launch(a, u, v))
+            for _ in range(6):
+                for _ in range(40):
+                    O.lib.orc_ks_step(a, C.byref(ks), 0.25)
+                bl = O.KerrState()
+                O.lib.orc_ks_to_bl(a, C.byref(ks), C.byref(bl))
+                want = (C.c_double * 3)()
+                O.lib.orc_kerr_position(a, C.byref(bl), want)
+                got = (C.c_double * 3)()
+                O.lib.orc_ks_position(a, C.byref(ks), got)
+                assert np.abs(np.array(got[:]) - np.array(want[:])).max() < 1e-9 * max(1.0, np.abs(want[:]).max())
