@@ -30,6 +30,7 @@ def run(*args, timeout=600):
     (["--particles"], 64, "--particles needs a value."),
     (["--bogus"], 64, "unknown argument --bogus."),
     (["serve", "--policy", "lifo"], 64, "unknown policy lifo; choose srpt or fifo."),
+    (["--kerr-schild", "--spin", "1.5"], 64, "spin must be between 0 and 0.998."),
     ([], 2, "macOS only"),
 ])
 def test_option_grammar_and_exit_codes(args, code, sentence):
