@@ -30,6 +30,7 @@ GENERATOR_DISK, GENERATOR_ANNULUS = 0, 1
 INTEGRATOR_REFERENCE, INTEGRATOR_FAST, INTEGRATOR_KERR_SCHILD = 0, 1, 2
 GARG_E_ARGUMENT, GARG_E_DEVICE, GARG_E_BUDGET, GARG_E_STATE = 1, 2, 3, 4
 FRAME_AUTO = 0xFFFFFFFF  # splat frame from the particle store's dispatch count
+PROBE_NOALLOC_MAX = 16384  # GARG_PROBE_NOALLOC_MAX: probes per garg_probe_noalloc call
 
 
 class GargError(RuntimeError):
@@ -131,6 +132,7 @@ SIGNATURES = {
     "garg_targets_reset_counters": (C.c_int, [vp, vp]),
     "garg_march_image": (C.c_int, [vp, vp, P(MarchUniforms), vp, vp, vp, u32, u32, u32]),
     "garg_probe": (C.c_int, [vp, vp, P(RayProbe), u32, P(MarchUniforms), vp, u32, P(RayProbeResult)]),
+    "garg_probe_noalloc": (C.c_int, [vp, vp, P(RayProbe), u32, P(MarchUniforms), vp, u32, P(RayProbeResult)]),
     "garg_request_rays": (u32, [u32, u32]),
     "garg_batch_config_default": (C.c_int, [vp, P(BatchConfig)]),
     "garg_march_batch": (C.c_int, [vp, vp, P(MarchRequest), u32, P(BatchConfig), P(MarchCounters)]),
@@ -294,14 +296,17 @@ class Context:
         _check(self.lib.garg_graph_end(self.h, stream, C.byref(h)))
         return Graph(self, h)
 
-    def probe(self, probes, u: MarchUniforms, tables=None, integrator: int = INTEGRATOR_FAST):
+    def probe(self, probes, u: MarchUniforms, tables=None, integrator: int = INTEGRATOR_FAST, noalloc: bool = False):
+        """probeRays; noalloc=True goes through garg_probe_noalloc (at most
+        PROBE_NOALLOC_MAX probes, staging in the library's fixed array)."""
         n = len(probes)
         arr = (RayProbe * n)()
         for i, (o, d) in enumerate(probes):
             arr[i].origin[:] = [float(x) for x in o]
             arr[i].direction[:] = [float(x) for x in d]
         out = (RayProbeResult * n)()
-        _check(self.lib.garg_probe(self.h, None, arr, n, C.byref(u), tables.h if tables else None, integrator, out))
+        fn = self.lib.garg_probe_noalloc if noalloc else self.lib.garg_probe
+        _check(fn(self.h, None, arr, n, C.byref(u), tables.h if tables else None, integrator, out))
         return out
 
     def march_image(self, u: MarchUniforms, targets, volume=None, tables=None, integrator: int = INTEGRATOR_FAST,

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,33 @@ int fail(int code, const char* fmt, ...) {
     } while (0)
 
 double max_impact_parameter(double r) { return r / std::sqrt(1.0 - R_SCHWARZSCHILD / r); }
+
+// Staging of garg_probe_noalloc: the uploaded probes, then the results, for at
+// most GARG_PROBE_NOALLOC_MAX rays. Every device holds its own copy of this
+// array until the process ends. In host code the name `scratch` is only a
+// host-side stand-in: the device address of each device's copy is looked up
+// once, with that device current, and kept in g_scratch_ptr[device]; host code
+// passes that pointer, never the array's name, to kernels and copies.
+constexpr uint64_t NOALLOC_PROBES_BYTES = ((uint64_t)GARG_PROBE_NOALLOC_MAX * sizeof(garg_ray_probe) + 255u) & ~255ull;
+constexpr uint64_t NOALLOC_SCRATCH_BYTES =
+    NOALLOC_PROBES_BYTES + (uint64_t)GARG_PROBE_NOALLOC_MAX * sizeof(garg_ray_probe_result);
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Looks up this device's copy of `scratch` on the first context created on
+// it; `device` must be current.
+int bind_noalloc_scratch(int device) {
+    if (device >= MAX_DEVICES) return GARG_OK;  // garg_probe_noalloc refuses on such a device
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return GARG_OK;
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "locating the probe staging array");
+    g_scratch_ptr[device] = p;
+    return GARG_OK;
+}
 
 }  // namespace
 
@@ -251,12 +279,13 @@ int garg_ctx_create(int device, uint64_t budget_bytes, garg_ctx** out) {
     ctx->budget = budget_bytes ? budget_bytes : (uint64_t)(0.9 * (double)free_b);
     ctx->info.budget_bytes = ctx->budget;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "creating the default stream");
+    int rc = bind_noalloc_scratch(device);
+    if (rc) return rc;
 
     std::vector<float> bb, cdf;
     build_blackbody_table(bb);
     build_inverse_cdf(cdf);
     ctx->temperature_scale = temperature_scale();
-    int rc;
     if ((rc = ctx_alloc(ctx, bb.size() * 4, (void**)&ctx->blackbody, "the blackbody table"))) return rc;
     if ((rc = ctx_alloc(ctx, cdf.size() * 4, (void**)&ctx->inverse_cdf, "the seeding table"))) return rc;
     HIP_TRY(hipMemcpy(ctx->blackbody, bb.data(), bb.size() * 4, hipMemcpyHostToDevice), "uploading the blackbody table");
@@ -849,11 +878,13 @@ int garg_march_image(garg_ctx* ctx, garg_stream* stream, const garg_march_unifor
     return GARG_OK;
 }
 
-int garg_probe(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
-               const garg_march_uniforms* uniforms, garg_tables* tables, uint32_t integrator,
-               garg_ray_probe_result* results) {
+namespace {
+
+int check_probes(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
+                 const garg_march_uniforms* uniforms, garg_tables* tables, uint32_t integrator,
+                 garg_ray_probe_result* results, const char* what) {
     REQUIRE(ctx && probes && results, "ctx, probes and results are required.");
-    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_probe");
+    REFUSE_WHILE_CAPTURING(ctx, stream, what);
     REQUIRE(count > 0, "no probes.");
     REQUIRE(integrator <= 2u, "unknown integrator %u.", integrator);
     REQUIRE(uniforms && uniforms->spin >= 0.0f && uniforms->spin < 1.0f, "spin must be in [0, 1).");
@@ -871,21 +902,20 @@ int garg_probe(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes,
                     "spinning rays are marched within r = %g; probe %u starts at r = %g.", KERR_MAX_RADIUS, i, (double)r);
         }
     }
-    uint64_t need = align256((uint64_t)count * sizeof(garg_ray_probe)) + (uint64_t)count * sizeof(garg_ray_probe_result);
-    if (need > ctx->probe_scratch_bytes) {
-        ctx_free(ctx, ctx->probe_scratch, ctx->probe_scratch_bytes);
-        ctx->probe_scratch = nullptr;
-        ctx->probe_scratch_bytes = 0;
-        int rc = ctx_alloc(ctx, need, &ctx->probe_scratch, "the probe buffers");
-        if (rc) return rc;
-        ctx->probe_scratch_bytes = need;
-    }
+    return GARG_OK;
+}
+
+// Uploads the probes to `staging`, marches them and reads the results back
+// from `staging` + results_offset; waits for the stream.
+int run_probes(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
+               const garg_march_uniforms* uniforms, garg_tables* tables, uint32_t integrator,
+               garg_ray_probe_result* results, void* staging, uint64_t results_offset) {
     hipStream_t s = stream_of(ctx, stream);
     ProbeArgs a;
     a.uniforms = device_uniforms(uniforms, nullptr);
     a.scene = make_scene(ctx, nullptr, tables);
-    a.probes = (const garg_ray_probe*)ctx->probe_scratch;
-    a.results = (garg_ray_probe_result*)((char*)ctx->probe_scratch + align256((uint64_t)count * sizeof(garg_ray_probe)));
+    a.probes = (const garg_ray_probe*)staging;
+    a.results = (garg_ray_probe_result*)((char*)staging + results_offset);
     a.count = count;
     HIP_TRY(hipMemcpyAsync((void*)a.probes, probes, (size_t)count * sizeof(garg_ray_probe), hipMemcpyHostToDevice, s),
             "uploading probes");
@@ -893,6 +923,40 @@ int garg_probe(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes,
     HIP_TRY(hipMemcpyAsync(results, a.results, (size_t)count * sizeof(garg_ray_probe_result), hipMemcpyDeviceToHost, s),
             "reading probe results");
     return sync_stream(s, "waiting for the probes");
+}
+
+}  // namespace
+
+int garg_probe(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
+               const garg_march_uniforms* uniforms, garg_tables* tables, uint32_t integrator,
+               garg_ray_probe_result* results) {
+    int rc = check_probes(ctx, stream, probes, count, uniforms, tables, integrator, results, "garg_probe");
+    if (rc) return rc;
+    uint64_t offset = align256((uint64_t)count * sizeof(garg_ray_probe));
+    uint64_t need = offset + (uint64_t)count * sizeof(garg_ray_probe_result);
+    if (need > ctx->probe_scratch_bytes) {
+        ctx_free(ctx, ctx->probe_scratch, ctx->probe_scratch_bytes);
+        ctx->probe_scratch = nullptr;
+        ctx->probe_scratch_bytes = 0;
+        rc = ctx_alloc(ctx, need, &ctx->probe_scratch, "the probe buffers");
+        if (rc) return rc;
+        ctx->probe_scratch_bytes = need;
+    }
+    return run_probes(ctx, stream, probes, count, uniforms, tables, integrator, results, ctx->probe_scratch, offset);
+}
+
+int garg_probe_noalloc(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
+                       const garg_march_uniforms* uniforms, garg_tables* tables, uint32_t integrator,
+                       garg_ray_probe_result* results) {
+    int rc = check_probes(ctx, stream, probes, count, uniforms, tables, integrator, results, "garg_probe_noalloc");
+    if (rc) return rc;
+    REQUIRE(count <= GARG_PROBE_NOALLOC_MAX,
+            "garg_probe_noalloc takes at most %u probes per call (its staging is a fixed array), not %u.",
+            GARG_PROBE_NOALLOC_MAX, count);
+    void* staging = ctx->device < MAX_DEVICES ? g_scratch_ptr[ctx->device] : nullptr;
+    if (!staging)
+        return fail(GARG_E_STATE, "garg_probe_noalloc has no staging array on device %d.", ctx->device);
+    return run_probes(ctx, stream, probes, count, uniforms, tables, integrator, results, staging, NOALLOC_PROBES_BYTES);
 }
 
 uint32_t garg_request_rays(uint32_t width, uint32_t height) { return ((width + 7u) / 8u) * ((height + 7u) / 8u) * 64u; }

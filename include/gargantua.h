@@ -266,6 +266,20 @@ int garg_probe(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes,
                const garg_march_uniforms* uniforms, garg_tables* tables, uint32_t integrator,
                garg_ray_probe_result* results);
 
+/* No counterpart in the reference. garg_probe for callers that cannot
+ * allocate device memory (tight validation or serving loops, harnesses that
+ * forbid hipMalloc): the same checks, kernel and results, but the device
+ * staging of the probes and results is a fixed array inside this library
+ * (1.1 MB per device, held until the process ends) instead of a context
+ * allocation, and the call allocates nothing. At most GARG_PROBE_NOALLOC_MAX
+ * probes per call (GARG_E_ARGUMENT beyond). Every call on one device shares
+ * that device's array: two calls that overlap on one device (from two host
+ * threads, or two contexts) race and corrupt each other's results. */
+#define GARG_PROBE_NOALLOC_MAX 16384u
+int garg_probe_noalloc(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
+                       const garg_march_uniforms* uniforms, garg_tables* tables, uint32_t integrator,
+                       garg_ray_probe_result* results);
+
 /* ---- batched persistent march (no reference counterpart; nearest is the
  * reference's one launch per frame, MarchPass.encodeFrame) ------------------ */
 
