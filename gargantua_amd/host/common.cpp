@@ -35,7 +35,8 @@ const char* kUsage =
     "usage: gargantua [--preset small|default|max] [--particles N] [--volume 96|128|192|256] [--seed S] "
     "[--strategy march|bake] [--fps-cap 30|60] [--no-upscale] [--spin A] [--kerr-schild] [--camera az,el,dist] "
     "[--still out.png --width W --height H [--sequence N]] | bench [--sweep] | validate | serve [--rate R] "
-    "[--seconds T] [--deadline D] [--frames F] [--tick-rays N] [--policy srpt|fifo] [--verify N] [--report PATH] | version | constants";
+    "[--seconds T] [--deadline D] [--frames F] [--tick-rays N] [--policy srpt|fifo] [--bake-share F] [--max-wait S] "
+    "[--verify N] [--report PATH] | version | constants";
 
 namespace {
 
@@ -180,6 +181,13 @@ std::string parse_options(const std::vector<std::string>& args, Options& o) {
             if (!(err = value(a, o.serve_policy)).empty()) return err;
             if (o.serve_policy != "srpt" && o.serve_policy != "fifo")
                 return "unknown policy " + o.serve_policy + "; choose srpt or fifo.";
+        } else if (a == "--bake-share") {
+            if (!(err = value(a, text)).empty()) return err;
+            if (!parse_double(text, o.serve_bake_share) || o.serve_bake_share < 0 || o.serve_bake_share > 1)
+                return "bake-share needs a number between 0 and 1.";
+        } else if (a == "--max-wait") {
+            if (!(err = value(a, text)).empty()) return err;
+            if (!parse_double(text, o.serve_max_wait) || o.serve_max_wait <= 0) return "max-wait needs a positive number.";
         } else if (a == "--report") {
             if (!(err = value(a, o.serve_report)).empty()) return err;
         } else {

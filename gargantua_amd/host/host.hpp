@@ -57,6 +57,12 @@ struct Options {
     // How a tick's ray budget is shared: "srpt" gives it to the active
     // requests with the least work left first, "fifo" in arrival order.
     std::string serve_policy = "srpt";
+    // With --strategy bake: the share of still requests that are static
+    // Schwarzschild stills (spin 0), which bake once and walk.
+    double serve_bake_share = 0.25;
+    // Shed a request at admission when the work already admitted plus its
+    // own, at the measured rate, would take longer than this (0 = never).
+    double serve_max_wait = 0.0;
 };
 
 extern const char* kUsage;

@@ -13,7 +13,11 @@
 // GPU join the next tick and share the device with them. A request pins the
 // newest volume version at admission and renders all its frames against it,
 // which makes its result independent of what it was batched with; --verify
-// re-renders requests alone and compares the bits.
+// re-renders requests alone (march, or bake and walk) and compares the bits.
+// With --max-wait, a request whose predicted wait (the admitted work left
+// plus its own, in RK4 steps, over the measured rate) exceeds the bound is
+// shed at admission instead of queueing behind more work than the GPU can
+// finish in time.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -63,6 +67,7 @@ struct Request {
     garg_targets* targets = nullptr;
     garg_bake* baked = nullptr;
     bool baked_ready = false;
+    bool shed = false;
     garg_march_uniforms uniforms{};
     uint64_t steps = 0, exhausted = 0, truncated = 0;
 };
@@ -89,6 +94,20 @@ garg_march_uniforms frame_uniforms(const Request& r) {
                                           true, 1u, true, r.spin, r.budget == STEP_CAP_STILL);
     u.geodesic.escape_radius = std::max(GARG_F(R_ESCAPE), (float)r.r_obs + 10.0f);
     return u;
+}
+
+// The walk takes the request's own uniforms with the bake spacing; the bake
+// the still policy and no jitter, as the reference bakes
+// (App/Renderer.swift:171-176), keeping the request's escape radius.
+void bake_uniforms(const Request& r, garg_tables* t, int volume_size, garg_march_uniforms& walk,
+                   garg_march_uniforms& bake) {
+    walk = r.uniforms;
+    check(garg_tables_apply(t, &walk), "applying tables");
+    walk.bake_spacing = (float)(2.0 * VOLUME_HALF_EXTENT / volume_size * BAKE_SPACING_VOXELS);
+    bake = walk;
+    bake.geodesic = policy_settings(Policy::still);
+    bake.geodesic.escape_radius = walk.geodesic.escape_radius;
+    bake.jitter[0] = bake.jitter[1] = 0.0f;
 }
 
 }  // namespace
@@ -153,7 +172,10 @@ void run_serve(const Options& o) {
         r.r_obs = 30.0 + 70.0 * U(rng);
         r.budget = U(rng) < 0.5 ? STEP_CAP_INTERACTIVE : STEP_CAP_STILL;
         r.frames = r.sequence ? o.serve_sequence_frames : 1;
-        // Static-camera Schwarzschild requests bake (the bake is Schwarzschild only).
+        // With the bake strategy a stated share of the stills are static
+        // Schwarzschild stills, and these bake: the bake is Schwarzschild only
+        // (the reference refuses spin with it, App/Renderer.swift:62-64).
+        if (!r.sequence && c.strategy == Strategy::bake && U(rng) < o.serve_bake_share) r.spin = 0.0;
         r.bake = !r.sequence && c.strategy == Strategy::bake && r.spin == 0.0;
         r.rays = garg_request_rays((uint32_t)r.w, (uint32_t)r.h);
         r.footprint = target_bytes(r.w, r.h) + (r.bake ? garg_bake_bytes(r.w, r.h, BAKE_SAMPLES) : 0);
@@ -172,14 +194,26 @@ void run_serve(const Options& o) {
     size_t next_arrival = 0;
     std::deque<int> pending;
     std::vector<int> active;
-    uint64_t in_use = 0, total_steps = 0, ticks = 0, refused = 0, verified = 0, mismatched = 0;
+    uint64_t in_use = 0, total_steps = 0, ticks = 0, refused = 0, shed = 0, verified = 0, mismatched = 0;
+    uint64_t verified_bakes = 0;
+    // Expected RK4 steps per ray (interactive, still budget), refined from
+    // the frames that finish, and the marching rate, for --max-wait.
+    double steps_per_ray[2] = {100.0, 200.0};
+    uint64_t seen_rays[2] = {0, 0}, seen_steps[2] = {0, 0};
+    double march_seconds = 0;
+    auto kind = [](const Request& r) { return r.budget > STEP_CAP_INTERACTIVE ? 1 : 0; };
+    auto work_left = [&](const Request& r) {
+        const uint64_t rays = r.bake ? (r.baked_ready ? 0 : r.rays)
+                                     : (uint64_t)(r.frames - r.frame - 1) * r.rays + (r.rays - r.issued);
+        return (double)rays * steps_per_ray[r.bake ? 1 : kind(r)];
+    };
     double gpu_busy = 0;
     const auto start = Clock::now();
     auto now_s = [&] { return std::chrono::duration<double>(Clock::now() - start).count(); };
     size_t completed = 0;
 
     double next_progress = 20.0;
-    while (completed + refused < all.size()) {
+    while (completed + refused + shed < all.size()) {
         double now = now_s();
         if (o.serve_deadline > 0 && now > o.serve_deadline) break;  // report what finished
         if (now >= next_progress) {
@@ -203,6 +237,18 @@ void run_serve(const Options& o) {
                 ++refused;
                 pending.pop_front();
                 continue;
+            }
+            if (o.serve_max_wait > 0) {
+                double ahead = work_left(r);
+                for (int id : active) ahead += work_left(all[id]);
+                const double rate = march_seconds > 0.5 ? total_steps / march_seconds : 5e10;
+                if (ahead / rate > o.serve_max_wait) {
+                    r.done = now;
+                    r.shed = true;
+                    ++shed;
+                    pending.pop_front();
+                    continue;
+                }
             }
             if (in_use + r.footprint > serving_budget || active.size() >= 200) break;
             check(garg_targets_create(s.ctx, (uint32_t)r.w, (uint32_t)r.h, &r.targets), "allocating targets");
@@ -274,20 +320,20 @@ void run_serve(const Options& o) {
             garg_march_counters total;
             check(garg_march_batch(s.ctx, nullptr, batch.data(), (uint32_t)batch.size(), &cfg, &total), "marching");
             total_steps += total.steps;
+            march_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
         }
         // Bake requests: bake once, then walk against their pinned volume.
         for (int id : active) {
             Request& r = all[id];
             if (!r.bake) continue;
-            garg_march_uniforms u = r.uniforms;
             garg_tables* t = tables_for((float)r.r_obs);
-            check(garg_tables_apply(t, &u), "applying tables");
-            u.bake_spacing = (float)(2.0 * VOLUME_HALF_EXTENT / c.volume * BAKE_SPACING_VOXELS);
+            garg_march_uniforms walk, bake;
+            bake_uniforms(r, t, c.volume, walk, bake);
             if (!r.baked_ready) {
-                check(garg_bake_rows(r.baked, nullptr, &u, t, 0, (uint32_t)r.h), "baking");
+                check(garg_bake_rows(r.baked, nullptr, &bake, t, 0, (uint32_t)r.h), "baking");
                 r.baked_ready = true;
             }
-            check(garg_bake_walk(r.baked, nullptr, &u, versions[r.volume_slot].volume, r.targets), "walking");
+            check(garg_bake_walk(r.baked, nullptr, &walk, versions[r.volume_slot].volume, r.targets), "walking");
             r.tick_issued = r.rays - r.issued;
         }
         check(garg_stream_sync(s.ctx, nullptr), "waiting for the tick");
@@ -306,6 +352,14 @@ void run_serve(const Options& o) {
             garg_march_counters cnt;
             check(garg_targets_read_counters(r.targets, &cnt), "reading counters");
             r.exhausted = cnt.exhausted;
+            if (!r.bake && cnt.rays > 0) {
+                // The counters accumulate over the request's frames.
+                const int k = kind(r);
+                seen_rays[k] += r.rays;
+                seen_steps[k] += cnt.steps - r.steps;
+                r.steps = cnt.steps;
+                steps_per_ray[k] = (double)seen_steps[k] / (double)seen_rays[k];
+            }
             if (r.baked) {
                 uint32_t over = 0;
                 check(garg_bake_overflow(r.baked, &over), "reading the bake overflow");
@@ -319,37 +373,53 @@ void run_serve(const Options& o) {
                 continue;
             }
             // Finished: verify alone (every Nth request), release.
-            if (o.serve_verify > 0 && r.id % o.serve_verify == 0 && !r.bake) {
-                std::vector<uint16_t> mixed((size_t)r.w * r.h * 8), alone((size_t)r.w * r.h * 8);
-                check(garg_targets_read(r.targets, mixed.data(), mixed.data() + (size_t)r.w * r.h * 4, nullptr),
-                      "reading");
+            if (o.serve_verify > 0 && r.id % o.serve_verify == 0) {
+                const size_t n = (size_t)r.w * r.h;
+                std::vector<uint16_t> mixed(n * 8), alone(n * 8);
+                std::vector<float> mixed_depth(n), alone_depth(n);
+                check(garg_targets_read(r.targets, mixed.data(), mixed.data() + n * 4, mixed_depth.data()), "reading");
                 garg_targets* t2 = nullptr;
                 check(garg_targets_create(s.ctx, (uint32_t)r.w, (uint32_t)r.h, &t2), "allocating");
-                garg_march_request q{};
-                q.uniforms = r.uniforms;
-                q.targets = t2;
-                q.volume = versions[r.volume_slot].volume;
-                if (r.spin == 0.0) {
-                    q.tables = tables_for((float)r.r_obs);
-                    check(garg_tables_apply(q.tables, &q.uniforms), "applying tables");
+                if (r.bake) {
+                    garg_tables* t = tables_for((float)r.r_obs);
+                    garg_march_uniforms walk, bake;
+                    bake_uniforms(r, t, c.volume, walk, bake);
+                    garg_bake* b2 = nullptr;
+                    check(garg_bake_create(s.ctx, (uint32_t)r.w, (uint32_t)r.h, BAKE_SAMPLES, &b2), "baking alone");
+                    check(garg_bake_rows(b2, nullptr, &bake, t, 0, (uint32_t)r.h), "baking alone");
+                    check(garg_bake_walk(b2, nullptr, &walk, versions[r.volume_slot].volume, t2), "walking alone");
+                    check(garg_stream_sync(s.ctx, nullptr), "waiting for the bake");
+                    garg_bake_destroy(b2);
+                } else {
+                    garg_march_request q{};
+                    q.uniforms = r.uniforms;
+                    q.targets = t2;
+                    q.volume = versions[r.volume_slot].volume;
+                    if (r.spin == 0.0) {
+                        q.tables = tables_for((float)r.r_obs);
+                        check(garg_tables_apply(q.tables, &q.uniforms), "applying tables");
+                    }
+                    check(garg_march_batch(s.ctx, nullptr, &q, 1, &cfg, nullptr), "marching alone");
                 }
-                check(garg_march_batch(s.ctx, nullptr, &q, 1, &cfg, nullptr), "marching alone");
-                check(garg_targets_read(t2, alone.data(), alone.data() + (size_t)r.w * r.h * 4, nullptr), "reading");
+                check(garg_targets_read(t2, alone.data(), alone.data() + n * 4, alone_depth.data()), "reading");
                 garg_targets_destroy(t2);
                 ++verified;
-                if (fnv(mixed.data(), mixed.size() * 2) != fnv(alone.data(), alone.size() * 2)) {
+                if (r.bake) ++verified_bakes;
+                if (fnv(mixed.data(), mixed.size() * 2, fnv(mixed_depth.data(), n * 4)) !=
+                    fnv(alone.data(), alone.size() * 2, fnv(alone_depth.data(), n * 4))) {
                     ++mismatched;
                     size_t differing = 0, first = 0;
-                    for (size_t k = 0; k < (size_t)r.w * r.h; ++k)
+                    for (size_t k = 0; k < n; ++k)
                         if (memcmp(&mixed[4 * k], &alone[4 * k], 8) != 0 ||
-                            memcmp(&mixed[(size_t)r.w * r.h * 4 + 4 * k], &alone[(size_t)r.w * r.h * 4 + 4 * k], 8) != 0)
+                            memcmp(&mixed[n * 4 + 4 * k], &alone[n * 4 + 4 * k], 8) != 0 ||
+                            memcmp(&mixed_depth[k], &alone_depth[k], 4) != 0)
                             if (differing++ == 0) first = k;
                     char msg[512];
                     snprintf(msg, sizeof msg,
-                             "mismatch: request %d %dx%d spin %.3f budget %u frames %d slices %d slot %d: %zu pixels "
+                             "mismatch: request %d %dx%d spin %.3f budget %u frames %d slices %d slot %d%s: %zu pixels "
                              "differ, first (%zu, %zu)",
-                             r.id, r.w, r.h, r.spin, r.budget, r.frames, r.frame_slices, r.volume_slot, differing,
-                             first % (size_t)r.w, first / (size_t)r.w);
+                             r.id, r.w, r.h, r.spin, r.budget, r.frames, r.frame_slices, r.volume_slot,
+                             r.bake ? " (baked)" : "", differing, first % (size_t)r.w, first / (size_t)r.w);
                     line(msg);
                 }
             }
@@ -366,10 +436,11 @@ void run_serve(const Options& o) {
     }
     double wall = now_s();
     std::vector<double> latency;
-    uint64_t exhausted = 0, truncated = 0;
+    uint64_t exhausted = 0, truncated = 0, baked = 0;
     size_t unfinished = 0;
     for (auto& r : all) {
         if (r.done < 0) ++unfinished;  // past the deadline: still queued or on the GPU
+        if (r.bake && r.done >= 0 && !r.shed && r.admitted >= 0) ++baked;
         if (r.admitted >= 0 && r.done >= 0) {
             latency.push_back(r.done - r.arrival);
             exhausted += r.exhausted;
@@ -382,17 +453,20 @@ void run_serve(const Options& o) {
     };
     char buf[1024];
     snprintf(buf, sizeof buf,
-             "{\"requests\": %zu, \"completed\": %zu, \"unfinished\": %zu, \"refused\": %llu, \"wall_s\": %.3f, "
+             "{\"requests\": %zu, \"completed\": %zu, \"unfinished\": %zu, \"refused\": %llu, \"shed\": %llu, "
+             "\"max_wait_s\": %.3f, \"baked\": %llu, \"wall_s\": %.3f, "
              "\"ticks\": %llu, "
              "\"latency_p50_s\": %.4f, \"latency_p99_s\": %.4f, \"latency_max_s\": %.4f, \"steps\": %llu, "
              "\"steps_per_s\": %.4e, \"gpu_busy_fraction\": %.3f, \"exhausted_rays\": %llu, \"truncated_rays\": %llu, "
-             "\"verified\": %llu, \"mismatched\": %llu, \"rate\": %.3f, \"arrival_seconds\": %.3f, \"frames\": %d, "
-             "\"policy\": \"%s\", \"kerr_schild\": %s}",
-             all.size(), completed, unfinished, (unsigned long long)refused, wall, (unsigned long long)ticks, pct(0.5),
+             "\"verified\": %llu, \"verified_baked\": %llu, \"mismatched\": %llu, \"rate\": %.3f, "
+             "\"arrival_seconds\": %.3f, \"frames\": %d, \"policy\": \"%s\", \"kerr_schild\": %s}",
+             all.size(), completed, unfinished, (unsigned long long)refused, (unsigned long long)shed, o.serve_max_wait,
+             (unsigned long long)baked, wall, (unsigned long long)ticks, pct(0.5),
              pct(0.99),
              latency.empty() ? 0.0 : latency.back(), (unsigned long long)total_steps, total_steps / wall,
              gpu_busy / wall, (unsigned long long)exhausted, (unsigned long long)truncated,
-             (unsigned long long)verified, (unsigned long long)mismatched, o.serve_rate, o.serve_seconds,
+             (unsigned long long)verified, (unsigned long long)verified_bakes, (unsigned long long)mismatched,
+             o.serve_rate, o.serve_seconds,
              o.serve_sequence_frames, o.serve_policy.c_str(), o.kerr_schild ? "true" : "false");
     line(buf);
     if (!o.serve_report.empty()) {

@@ -192,6 +192,15 @@ _sig("orc_render", None, C.c_int, C.c_int, P(Camera), P(Field), P(Settings), d, 
 _sig("orc_march_pixel", None, P(MarchParams), P(Field), C.c_int, C.c_int, P(Pixel))
 _sig("orc_march_pixels", i64, P(MarchParams), P(Field), P(C.c_int32), i64, C.c_int, P(Pixel))
 _sig("orc_probe", None, P(MarchParams), dp, dp, dp, dp, dp, P(C.c_int), P(i64), dp, dp)
+
+
+class BakedRay(C.Structure):
+    _fields_ = [("count", i64), ("falls_in", C.c_int), ("overflowed", C.c_int), ("direction", d * 3),
+                ("depth", d)]
+
+
+_sig("orc_bake_pixel", None, P(MarchParams), C.c_int, C.c_int, i64, d, dp, P(BakedRay))
+_sig("orc_walk_pixel", None, P(MarchParams), P(Field), dp, i64, C.c_int, dp, dp)
 _sig("orc_starfield", None, dp, u32, dp, dp)
 _sig("orc_pbr_neutral", None, dp, dp)
 _sig("orc_splat_replay", None, vp, u32, u32, C.c_float, C.c_float, C.c_float, C.c_float, u32, u32, u64,

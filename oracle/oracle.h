@@ -242,6 +242,26 @@ void orc_probe(const orc_march_params* p, const double origin[3], const double d
                double position_out[3], double direction_out[3], double* drift, int* outcome, int64_t* steps,
                double* energy, double* angular_momentum);
 void orc_starfield(const double direction[3], uint32_t seed, const double* blackbody, double out[3]);
+
+/* ---- Bake and walk (bake.metal:17-92, 94-144) ---------------------------- */
+typedef struct {
+    int64_t count;          /* samples stored (at most the capacity) */
+    int falls_in;
+    int overflowed;         /* a sample fell beyond the capacity */
+    double direction[3];    /* sky direction of the ray's end */
+    double depth;           /* min(depth / (2 R_ESCAPE), 1): path to the first sample, or all of it */
+} orc_baked_ray;
+/* bakeRays for one pixel with the Schwarzschild march of p (spin 0):
+ * samples[4 k .. 4 k + 3] = (position, path since the previous sample) of
+ * sample k, for k < header->count; `capacity` and `spacing` are the
+ * uniforms' bakeCapacity and bakeSpacing. */
+void orc_bake_pixel(const orc_march_params* p, int px, int py, int64_t capacity, double spacing, double* samples,
+                    orc_baked_ray* header);
+/* walkRays for one pixel: composite `count` stored samples against `field`
+ * from the camera position of p, then the starfield along `direction` unless
+ * the ray falls in. */
+void orc_walk_pixel(const orc_march_params* p, const orc_field* field, const double* samples, int64_t count,
+                    int falls_in, const double direction[3], double color_out[3]);
 void orc_pbr_neutral(const double in[3], double out[3]);
 
 /* ---- Exact replays of the fp32 integer deposit (splat.metal:16-88) ------ */

@@ -528,4 +528,129 @@ void orc_probe(const orc_march_params* p, const double origin[3], const double d
     *angular_momentum = orc_schw_angular_momentum(&m.ray.s);
 }
 
+// bake.metal:17-92: the march of marchRay without the gather, storing the
+// in-slab points at least `spacing` of path apart.
+void orc_bake_pixel(const orc_march_params* p, int px, int py, int64_t capacity, double spacing, double* samples,
+                    orc_baked_ray* header) {
+    double sx = (double)px + 0.5, sy = (double)py + 0.5;
+    double nx = (2.0 * sx / (double)p->width - 1.0) * p->tan_half_fov[0];
+    double ny = (1.0 - 2.0 * sy / (double)p->height) * p->tan_half_fov[1];
+    v3 dir = orc::normalize(orc::make(p->camera_forward) + nx * orc::make(p->camera_right) + ny * orc::make(p->camera_up));
+    double direction[3];
+    orc::store(dir, direction);
+    SchwMarch m = prepare(p, p->camera_position, direction);
+    header->count = 0;
+    header->falls_in = 0;
+    header->overflowed = 0;
+    if (m.sky_only) {
+        orc::store(m.sky, header->direction);
+        header->depth = 1.0;
+        return;
+    }
+    orc_ray& ray = m.ray;
+    const orc_settings* g = &p->geodesic;
+    double prev[3];
+    orc_schw_position(&ray, prev);
+    double since = 0.0, travelled = 0.0, first_depth = -1.0;
+    int outcome = 2;
+    bool left = false;
+    for (int64_t k = 0; k < g->step_cap; ++k) {
+        double h = step_length(g, ray.s.r, prev[2]);
+        orc_ray_state before = ray.s;
+        orc_schw_step(&ray.s, h);
+        if (p->sphere_radius > 0.0 && ray.s.r >= p->sphere_radius && ray.s.rdot > 0.0 && before.r < p->sphere_radius) {
+            double frac = (p->sphere_radius - before.r) / (ray.s.r - before.r);
+            ray.s = before;
+            orc_schw_step(&ray.s, h * frac);
+        }
+        double pos[3];
+        orc_schw_position(&ray, pos);
+        double path = orc::length(orc::make(pos) - orc::make(prev));
+        since += path;
+        travelled += path;
+        prev[0] = pos[0];
+        prev[1] = pos[1];
+        prev[2] = pos[2];
+        if (ray.s.r <= g->capture_radius) {
+            outcome = 0;
+            break;
+        }
+        if (std::fabs(pos[2]) < DISK_SLAB_HALF_HEIGHT && std::fabs(pos[0]) < p->volume_half_extent &&
+            std::fabs(pos[1]) < p->volume_half_extent && since >= spacing) {
+            if (header->count < capacity) {
+                double* s = samples + 4 * header->count;
+                s[0] = pos[0];
+                s[1] = pos[1];
+                s[2] = pos[2];
+                s[3] = since;
+                header->count += 1;
+                if (first_depth < 0.0) first_depth = travelled;
+            } else {
+                header->overflowed = 1;
+            }
+            since = 0.0;
+        }
+        if (p->sphere_radius > 0.0 && ray.s.r >= p->sphere_radius && ray.s.rdot > 0.0) {
+            outcome = 1;
+            left = true;
+            break;
+        }
+        if (ray.s.r >= g->escape_radius) {
+            outcome = 1;
+            break;
+        }
+    }
+    header->falls_in = falls_in(ray, outcome) ? 1 : 0;
+    v3 fin;
+    if (left) fin = sphere_exit_direction(p, ray);
+    else {
+        double d[3];
+        orc_schw_direction(&ray, d);
+        fin = orc::make(d);
+    }
+    orc::store(fin, header->direction);
+    double depth = first_depth >= 0.0 ? first_depth : travelled;
+    header->depth = std::min(depth / (2.0 * R_ESCAPE), 1.0);
+}
+
+// bake.metal:94-144: the stored samples against the current field, with the
+// full Doppler factor cubed (no density blend) at f(|position|) along the
+// chord from the previous sample, then the starfield.
+void orc_walk_pixel(const orc_march_params* p, const orc_field* field, const double* samples, int64_t count,
+                    int falls_in_flag, const double direction[3], double color_out[3]) {
+    v3 color = {0, 0, 0};
+    double transmittance = 1.0;
+    v3 prev = orc::make(p->camera_position);
+    for (int64_t k = 0; k < count; ++k) {
+        if (transmittance <= TRANSMITTANCE_FLOOR) break;
+        const double* s = samples + 4 * k;
+        v3 pos = {s[0], s[1], s[2]};
+        double path = s[3];
+        if (field) {
+            double e[4], v[3];
+            orc_field_sample(field, s, e, v);
+            if (e[3] > 0.0) {
+                double g3 = 1.0;
+                if (p->redshift) {
+                    double d[3];
+                    orc::store(orc::normalize(pos - prev), d);
+                    double g = orc_redshift_factor(1.0 - R_SCHWARZSCHILD / orc::length(pos), d, v);
+                    g3 = g * g * g;
+                }
+                double alpha = 1.0 - std::exp(-p->opacity_scale * e[3] * path);
+                color = color + (transmittance * g3 * path) * v3{e[0], e[1], e[2]};
+                transmittance *= 1.0 - alpha;
+            }
+        }
+        prev = pos;
+    }
+    if (!falls_in_flag && p->star_brightness > 0.0) {
+        double d[3], star[3];
+        orc::store(orc::normalize(orc::make(direction)), d);
+        orc_starfield(d, p->star_seed, p->blackbody, star);
+        color = color + (transmittance * p->star_brightness) * orc::make(star);
+    }
+    orc::store(color, color_out);
+}
+
 }  // extern "C"

@@ -30,6 +30,8 @@ def run(*args, timeout=600):
     (["--particles"], 64, "--particles needs a value."),
     (["--bogus"], 64, "unknown argument --bogus."),
     (["serve", "--policy", "lifo"], 64, "unknown policy lifo; choose srpt or fifo."),
+    (["serve", "--bake-share", "1.5"], 64, "bake-share needs a number between 0 and 1."),
+    (["serve", "--max-wait", "0"], 64, "max-wait needs a positive number."),
     (["--kerr-schild", "--spin", "1.5"], 64, "spin must be between 0 and 0.998."),
     ([], 2, "macOS only"),
 ])
@@ -110,3 +112,53 @@ def test_serve_results_equal_alone_and_in_the_mix(tmp_path, tick_rays, policy, k
     assert rep["completed"] > 0 and rep["verified"] > 0 and rep["mismatched"] == 0
     assert rep["latency_p99_s"] >= rep["latency_p50_s"] > 0
     assert rep["policy"] == policy and rep["kerr_schild"] == kerr_schild
+
+
+@pytest.mark.gpu
+def test_serve_bakes_static_stills_and_verifies_them_alone(tmp_path):
+    """With --strategy bake every still of this stream is a static
+    Schwarzschild still (--bake-share 1): it bakes once with the still policy
+    and walks; re-baked and walked alone it gives the same bits."""
+    report = tmp_path / "serve.json"
+    r = run("serve", "--preset", "small", "--strategy", "bake", "--bake-share", "1", "--rate", "6", "--seconds", "3",
+            "--frames", "3", "--verify", "1", "--report", str(report))
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rep = json.loads(report.read_text())
+    assert rep["baked"] > 0 and rep["verified_baked"] > 0 and rep["mismatched"] == 0
+    assert rep["completed"] == rep["requests"] - rep["refused"] - rep["shed"]
+
+
+@pytest.mark.gpu
+def test_serve_max_preset_at_twenty_requests_per_second(tmp_path):
+    """The serving configuration on its own disk state, shortened: the max
+    preset (4M particles, 256^3) advancing every tick, Poisson arrivals at 20
+    req/s for 10 s, stills and 8-frame orbits of every resolution, spin and
+    budget, a quarter of the stills baked (up to 96 samples per ray, so some
+    rays truncate at 256^3's finer spacing), every fifth request re-rendered
+    alone and compared bit for bit."""
+    report = tmp_path / "serve.json"
+    r = run("serve", "--preset", "max", "--strategy", "bake", "--rate", "20", "--seconds", "10", "--frames", "8",
+            "--verify", "5", "--report", str(report))
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rep = json.loads(report.read_text())
+    assert rep["requests"] > 100 and rep["completed"] == rep["requests"] and rep["unfinished"] == 0
+    assert rep["verified"] >= rep["requests"] // 5 and rep["mismatched"] == 0
+    assert rep["baked"] > 0 and rep["truncated_rays"] > 0
+    assert rep["latency_p99_s"] >= rep["latency_p50_s"] > 0 and rep["steps_per_s"] > 1e10
+
+
+@pytest.mark.gpu
+def test_serve_sheds_what_it_cannot_finish_in_time(tmp_path):
+    """240-frame orbits at 20 req/s offer more than the GPU marches; with
+    --max-wait the loop sheds at admission what would wait longer, and
+    finishes everything it admits."""
+    report = tmp_path / "serve.json"
+    r = run("serve", "--preset", "small", "--rate", "20", "--seconds", "4", "--frames", "240", "--max-wait", "2",
+            "--report", str(report))
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rep = json.loads(report.read_text())
+    assert rep["shed"] > 0 and rep["unfinished"] == 0
+    assert rep["completed"] + rep["shed"] + rep["refused"] == rep["requests"]

@@ -348,6 +348,113 @@ def test_bake_walk_matches_march(ctx, settled_volume):
         o.close()
 
 
+def _baked(ctx, w, h, capacity=96, stars=True, redshift=True):
+    cam = api.orbit_camera()
+    tables = api.Tables(ctx)
+    tables.update(cam["distance"])
+    u = api.uniforms(cam, w, h, api.settings("still"), stars=stars, redshift=redshift, star_seed=3,
+                     bake_spacing=60.0 / 128 * 0.5)
+    tables.apply(u)
+    bake = api.Bake(ctx, w, h, capacity)
+    bake.rows(u, tables, 0, h)
+    p = scenes.oracle_params(u, scenes.oracle_tables(cam["distance"]))
+    return u, tables, bake, p
+
+
+def _unpack_half2(words):
+    h = np.stack([words & 0xFFFF, words >> 16], -1).astype(np.uint16)
+    return api.half_to_float(h).astype(np.float64)
+
+
+def test_bake_headers_and_samples_match_the_oracle(ctx):
+    """bakeRays (bake.metal:17-92) against the FP64 restatement per ray:
+    sample counts, falls-in flags, overflow, the packed sky direction and
+    depth, and the stored half4 points and path lengths. A point sits where
+    `path since the last point >= spacing` first holds, so an FP32 path that
+    crosses the threshold a step apart from FP64 shifts that ray's later
+    points by one step; such rays are counted, not compared."""
+    w, h = 192, 108
+    u, tables, bake, p = _baked(ctx, w, h)
+    headers, samples = bake.read()
+    count = headers[:, 0].astype(np.int64)
+    falls = headers[:, 1]
+    xy, zd = _unpack_half2(headers[:, 2]), _unpack_half2(headers[:, 3])
+    direction = np.concatenate([xy, zd[:, :1]], 1)
+    rows = range(0, h, 2)
+    same_count = same_falls = compared = aligned = 0
+    worst_dir = worst_depth = 0.0
+    over = 0
+    for y in rows:
+        for x in range(0, w, 2):
+            i = y * w + x
+            want = np.zeros(4 * 96)
+            hdr = O.BakedRay()
+            O.lib.orc_bake_pixel(C.byref(p), x, y, 96, float(u.bake_spacing), O.ptr(want), C.byref(hdr))
+            compared += 1
+            over += hdr.overflowed
+            same_falls += int(bool(falls[i]) == bool(hdr.falls_in))
+            if count[i] != hdr.count:
+                assert abs(int(count[i]) - hdr.count) <= 3, (x, y, count[i], hdr.count)
+                continue
+            same_count += 1
+            worst_depth = max(worst_depth, abs(zd[i, 1] - hdr.depth))
+            if not hdr.falls_in:
+                worst_dir = max(worst_dir, np.abs(direction[i] - np.array(hdr.direction[:])).max())
+            got = samples[:hdr.count, i, :].astype(np.float64)
+            ref = want.reshape(96, 4)[:hdr.count]
+            # half4 storage: half an ulp of |x| < 32 is 0.0078, of a path < 2 is 0.001.
+            if hdr.count == 0 or (np.abs(got[:, :3] - ref[:, :3]).max() < 0.02 and
+                                  np.abs(got[:, 3] - ref[:, 3]).max() < 0.01):
+                aligned += 1
+    print(f"bake vs oracle over {compared} rays: counts {same_count / compared:.4f}, falls-in "
+          f"{same_falls / compared:.4f}, points aligned {aligned / compared:.4f}, direction {worst_dir:.2e}, "
+          f"depth {worst_depth:.2e}, oracle overflow {over}, GPU overflow {bake.overflow()} of {w * h}")
+    assert same_count / compared > 0.97 and same_falls / compared > 0.995 and aligned / compared > 0.95
+    assert worst_dir < 4e-3 and worst_depth < 2e-3
+    assert count.max() <= 96
+    assert abs(bake.overflow() / (w * h) - over / compared) < 0.01
+    bake.close()
+    tables.close()
+
+
+def test_walk_matches_the_oracle_walk_of_the_same_samples(ctx, settled_volume):
+    """walkRays (bake.metal:94-144) against the FP64 walk of the very points
+    the GPU stored, on the GPU's own resolved volume, with redshift and stars:
+    colours per pixel within the half-precision output's rounding."""
+    w, h = 192, 108
+    u, tables, bake, p = _baked(ctx, w, h)
+    walked = api.Targets(ctx, w, h)
+    bake.walk(u, settled_volume, walked)
+    color = walked.read()[0].astype(np.float64)
+    headers, samples = bake.read()
+    field = scenes.field_from_volume(settled_volume)
+    xy, zd = _unpack_half2(headers[:, 2]), _unpack_half2(headers[:, 3])
+    errs = []
+    peak = np.abs(color[..., :3]).max()
+    lit = 0
+    for y in range(0, h, 2):
+        for x in range(0, w, 2):
+            i = y * w + x
+            n = int(headers[i, 0])
+            pts = np.ascontiguousarray(samples[:n, i, :], dtype=np.float64)
+            direction = (C.c_double * 3)(xy[i, 0], xy[i, 1], zd[i, 0])
+            out = (C.c_double * 3)()
+            O.lib.orc_walk_pixel(C.byref(p), C.byref(field), O.ptr(pts) if n else None, n, int(headers[i, 1]),
+                                 direction, out)
+            ref = np.array(out[:])
+            lit += int(ref.max() > 0)
+            errs.append((np.abs(color[y, x, :3] - ref) / np.maximum(np.abs(ref), 1e-2 * peak)).max())
+    errs = np.array(errs)
+    # A star whose cell boundary FP32 and FP64 place on opposite sides of a
+    # ray differs whole, so the bound is on the 99.8th percentile and the mean.
+    print(f"walk vs FP64 walk: 99.8% {np.quantile(errs, 0.998):.2e}, mean {errs.mean():.2e}, worst {errs.max():.2e} "
+          f"of the colour (floor 1% of the peak), {lit} lit pixels")
+    assert lit > 1000 and np.quantile(errs, 0.998) < 4e-3 and errs.mean() < 5e-4
+    np.testing.assert_array_equal(walked.read()[2], zd[:, 1].reshape(h, w).astype(np.float32))
+    for o in (bake, walked, tables):
+        o.close()
+
+
 def test_tonemap_matches_restatement(ctx):
     vol = disk_volume(ctx)
     req = kerr_request(ctx, 96, 54, 0.9, 60.0, 50.0, 1024, vol)
