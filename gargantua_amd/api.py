@@ -27,7 +27,7 @@ GARG_OK = 0
 RAY_CAPTURED, RAY_ESCAPED, RAY_EXHAUSTED = 0, 1, 2
 POTENTIAL_NEWTONIAN, POTENTIAL_PACZYNSKI_WIITA = 0, 1
 GENERATOR_DISK, GENERATOR_ANNULUS = 0, 1
-INTEGRATOR_REFERENCE, INTEGRATOR_FAST, INTEGRATOR_KERR_SCHILD = 0, 1, 2
+INTEGRATOR_REFERENCE, INTEGRATOR_FAST, INTEGRATOR_KERR_SCHILD, INTEGRATOR_POLAR = 0, 1, 2, 3
 GARG_E_ARGUMENT, GARG_E_DEVICE, GARG_E_BUDGET, GARG_E_STATE = 1, 2, 3, 4
 FRAME_AUTO = 0xFFFFFFFF  # splat frame from the particle store's dispatch count
 PROBE_NOALLOC_MAX = 16384  # GARG_PROBE_NOALLOC_MAX: probes per garg_probe_noalloc call

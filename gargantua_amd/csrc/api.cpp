@@ -855,7 +855,7 @@ int garg_march_image(garg_ctx* ctx, garg_stream* stream, const garg_march_unifor
     REQUIRE(ctx && targets, "ctx and targets are required.");
     int rc = check_uniforms(uniforms);
     if (rc) return rc;
-    REQUIRE(integrator <= 2u, "unknown integrator %u.", integrator);
+    REQUIRE(integrator <= 3u, "unknown integrator %u.", integrator);
     REQUIRE(targets->width == uniforms->resolution[0] && targets->height == uniforms->resolution[1],
             "the targets are %ux%u but the uniforms ask for %ux%u.", targets->width, targets->height,
             uniforms->resolution[0], uniforms->resolution[1]);
@@ -886,7 +886,7 @@ int check_probes(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probe
     REQUIRE(ctx && probes && results, "ctx, probes and results are required.");
     REFUSE_WHILE_CAPTURING(ctx, stream, what);
     REQUIRE(count > 0, "no probes.");
-    REQUIRE(integrator <= 2u, "unknown integrator %u.", integrator);
+    REQUIRE(integrator <= 3u, "unknown integrator %u.", integrator);
     REQUIRE(uniforms && uniforms->spin >= 0.0f && uniforms->spin < 1.0f, "spin must be in [0, 1).");
     REQUIRE(uniforms->geodesic.step_cap <= (1u << 24), "the step cap is above 2^24.");
     REQUIRE(uniforms->spin > 0.0f || uniforms->sphere_radius <= 0.0f || tables,
@@ -989,7 +989,7 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
     REQUIRE(config->block == 64 || config->block == 128 || config->block == 256, "block must be 64, 128 or 256.");
     REQUIRE(config->compaction_period > 0, "the compaction period must be positive.");
     REQUIRE(config->refill_threshold <= 64, "the refill threshold is a lane count of at most 64.");
-    REQUIRE(config->kerr_schild <= 1u, "kerr_schild is 0 or 1, not %u.", config->kerr_schild);
+    REQUIRE(config->kerr_schild <= 2u, "kerr_schild is 0, 1 or 2, not %u.", config->kerr_schild);
     for (uint32_t i = 0; i < count; ++i) {
         const garg_march_request& r = requests[i];
         int rc = check_uniforms(&r.uniforms);
@@ -1072,7 +1072,10 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
         BatchArgs a;
         a.requests = table;
         a.request_count = n;
-        a.kind = !kerr ? BATCH_SCHWARZSCHILD : config->kerr_schild ? BATCH_KERR_SCHILD : BATCH_KERR;
+        a.kind = !kerr                       ? BATCH_SCHWARZSCHILD
+                 : config->kerr_schild == 1u ? BATCH_KERR_SCHILD
+                 : config->kerr_schild == 2u ? BATCH_KERR_POLAR
+                                             : BATCH_KERR;
         a.total_rays = queue;
         a.queue = ctx->queue + (size_t)part * 8 * 16;
         a.totals = ctx->batch_totals;

@@ -736,6 +736,10 @@ struct KerrLoop {
     uint32_t redshift;
     uint32_t compensated;
     uint32_t gather;  // sample the volume (non-zero only when there is one)
+    // The request's spin and the constants of the Kerr-Schild form that
+    // follow from it: a^2, the horizons r+ and r-, a / (r+ - r-). Formed
+    // here, so that a wave-uniform loop holds them in scalar registers.
+    float spin, a2, rp, rm, shift_scale;
     VolumeView volume;
 };
 
@@ -749,6 +753,12 @@ GARG_DEV KerrLoop kerr_loop(const garg_march_uniforms& u, const Scene& sc, bool 
     l.redshift = u.redshift;
     l.compensated = compensated ? 1u : 0u;
     l.gather = gather && sc.volume.voxels != nullptr ? 1u : 0u;
+    l.spin = u.spin;
+    l.a2 = u.spin * u.spin;
+    const float root = sqrtf(fmaxf(1.0f - l.a2, 0.0f));
+    l.rp = 1.0f + root;
+    l.rm = 1.0f - root;
+    l.shift_scale = root > 0.0f ? u.spin / (l.rp - l.rm) : 0.0f;
     l.volume = sc.volume;
     return l;
 }
@@ -788,6 +798,31 @@ struct KerrFastMarch {
         done = false;
     }
 
+    // Continue a ray from state `s` at the point it has reached: the drift
+    // reference, the drift so far, the step count and the gather carry over.
+    GARG_DEV void resume(float spin, const KerrRayState& s, float carter_ref, float scale_inv, const Gather& gathered,
+                         float drift_so_far, uint32_t steps_so_far) {
+        k = kerr_consts(spin, s.energy, s.angular_momentum);
+        r = s.r;
+        th = s.theta;
+        ph = s.phi;
+        pr = s.pr;
+        pth = s.ptheta;
+        cr = cth = cph = cpr = cpth = 0.0f;
+        e = kerr_eval(k, r, th, pr, pth);
+        carter0 = carter_ref;
+        inv_scale = scale_inv;
+        capture = k.rplus + GARG_F(KERR_CAPTURE_MARGIN);
+        g = gathered;
+        prev_rho = fast_sqrt(r * r + k.a2) * e.st;
+        prev_z = r * e.ct;
+        prev_ph = ph;
+        drift = drift_so_far;
+        steps = steps_so_far;
+        outcome = GARG_RAY_EXHAUSTED;
+        done = false;
+    }
+
     GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget) {
         run_loop<false>(kerr_loop(u, sc, gather, compensated), budget);
     }
@@ -799,13 +834,22 @@ struct KerrFastMarch {
     template <bool UNIFORM, class Tick = NoTick>
     GARG_DEV void run_loop(KerrLoop l, uint32_t budget, Tick tick = Tick{}) {
         if constexpr (UNIFORM) l = wave_uniform(l);
-        const bool has_volume = l.gather != 0u;
-        const bool compensated = l.compensated != 0u;
         for (uint32_t n = 0; n < budget && !done; ++n) {
             tick(n);
+            step(l);
+        }
+        if (!done && steps >= l.step_cap) done = true;
+    }
+
+    // One RK4 step with its drift gauges, path and gather; sets `done` at
+    // the cap, at capture and at escape.
+    GARG_DEV void step(const KerrLoop& l) {
+        const bool has_volume = l.gather != 0u;
+        const bool compensated = l.compensated != 0u;
+        {
             if (steps >= l.step_cap) {
                 done = true;
-                break;
+                return;
             }
             float near_h = 0.2f * (r - k.rplus);
             float pole = fminf(fmaxf(e.st * (1.0f / GARG_F(KERR_POLE_SINE)), 0.05f), 1.0f);
@@ -841,7 +885,7 @@ struct KerrFastMarch {
             if (r <= capture) {
                 outcome = GARG_RAY_CAPTURED;
                 done = true;
-                break;
+                return;
             }
             if (has_volume && fabsf(z) < GARG_F(DISK_SLAB_HALF_HEIGHT)) {
                 float sp, cp;
@@ -859,10 +903,9 @@ struct KerrFastMarch {
             if (r >= l.escape_radius) {
                 outcome = GARG_RAY_ESCAPED;
                 done = true;
-                break;
+                return;
             }
         }
-        if (!done && steps >= l.step_cap) done = true;
     }
 
     GARG_DEV bool falls_in() const {
@@ -904,6 +947,21 @@ GARG_DEV float ks_radius(float a2, float x, float y, float z) {
 }
 
 GARG_DEV float ks_log_ratio(const KsConsts& k, float r) { return __logf(fabsf((r - k.rp) / (r - k.rm))); }
+
+// The constants of a ray: the spin's (the same for every ray of a request)
+// and the ray's energy and launch phase.
+GARG_DEV KsConsts ks_consts(float spin, float E, float phase) {
+    KsConsts k;
+    k.a = spin;
+    k.a2 = spin * spin;
+    k.E = E;
+    const float root = sqrtf(fmaxf(1.0f - k.a2, 0.0f));
+    k.rp = 1.0f + root;
+    k.rm = 1.0f - root;
+    k.shift_scale = root > 0.0f ? spin / (k.rp - k.rm) : 0.0f;
+    k.phase = phase;
+    return k;
+}
 
 // Hamilton's equations at a state, with the pieces of the metric that the
 // drift gauges reuse.
@@ -1006,10 +1064,12 @@ GARG_DEV KsState ks_from_bl(const KsConsts& k, const KerrRayState& s) {
 // is what the length of a step's chord needs; the absolute rotation is
 // formed only where the gas is sampled and at the end of the ray.
 struct KerrSchildMarch {
-    KsConsts k;
-    KsState s, c;   // state and TwoSum carries
+    // Per ray: the spin (for the finishing calls), the energy and the launch
+    // phase; the loop takes the spin's constants from its scalar parameters.
+    float a, E, phase;
+    KsState s, c;   // state and Fast2Sum carries
     KsEval e;       // evaluation at the current state
-    float carter0, l0, inv_scale, inv_root_scale, capture;
+    float carter0, l0, inv_scale, inv_root_scale;
     Gather g;
     f3 prev;        // Kerr-Schild position of the previous point
     float prev_r;
@@ -1018,24 +1078,22 @@ struct KerrSchildMarch {
     uint32_t outcome;
     bool done;
 
+    GARG_DEV KsConsts consts() const { return ks_consts(a, E, phase); }
+
     GARG_DEV void begin(float spin, const KerrRayState& launched) {
-        k.a = spin;
-        k.a2 = spin * spin;
-        k.E = launched.energy;
-        const float root = sqrtf(fmaxf(1.0f - k.a2, 0.0f));
-        k.rp = 1.0f + root;
-        k.rm = 1.0f - root;
-        k.shift_scale = root > 0.0f ? spin / (k.rp - k.rm) : 0.0f;
-        k.phase = ks_log_ratio(k, launched.r);
+        a = spin;
+        E = launched.energy;
+        KsConsts k = ks_consts(spin, E, 0.0f);
+        phase = ks_log_ratio(k, launched.r);
+        k.phase = phase;
         s = ks_from_bl(k, launched);
         c = KsState{0, 0, 0, 0, 0, 0};
         e = ks_eval(k, s);
         l0 = s.x * s.py - s.y * s.px;
-        carter0 = carter();
-        const float scale = fmaxf(fabsf(carter0), k.E * k.E);
+        carter0 = carter(k);
+        const float scale = fmaxf(fabsf(carter0), E * E);
         inv_scale = 1.0f / scale;
         inv_root_scale = 1.0f / sqrtf(scale);
-        capture = k.rp + GARG_F(KERR_CAPTURE_MARGIN);
         g.reset();
         prev = f3{s.x, s.y, s.z};
         prev_r = e.r;
@@ -1051,7 +1109,7 @@ struct KerrSchildMarch {
     // p_theta = (cos(theta) (r^2 + a^2) (x p_x + y p_y) - r rho^2 p_z)
     // / (rho sqrt(r^2 + a^2)). Near the axis, 1 - cos^2(theta) would lose
     // the small sin^2(theta) that L^2 / sin^2(theta) divides by.
-    GARG_DEV float carter() const {
+    GARG_DEV float carter(const KsConsts& k) const {
         const float rho2 = fmaxf(s.x * s.x + s.y * s.y, 1e-30f);
         const float ct = s.z * e.ir;
         const float rs = __builtin_amdgcn_rsqf(rho2 * e.q);  // 1 / (rho sqrt(r^2 + a^2))
@@ -1063,7 +1121,7 @@ struct KerrSchildMarch {
     }
 
     // cos and sin of psi at the current point.
-    GARG_DEV void rotation(float* cpsi, float* spsi) const {
+    GARG_DEV void rotation(const KsConsts& k, float* cpsi, float* spsi) const {
         const float rq = __builtin_amdgcn_rsqf(e.q);
         const float ca = e.r * rq, sa = k.a * rq;
         float sx, cx;
@@ -1072,16 +1130,16 @@ struct KerrSchildMarch {
         *spsi = sa * cx + ca * sx;
     }
 
-    GARG_DEV f3 position_bl() const {
+    GARG_DEV f3 position_bl(const KsConsts& k) const {
         float cs, sn;
-        rotation(&cs, &sn);
+        rotation(k, &cs, &sn);
         return f3{cs * s.x + sn * s.y, cs * s.y - sn * s.x, s.z};
     }
 
     // The locally measured direction in the Boyer-Lindquist frame (the
     // convention of kerr_direction) from the coordinate velocities at the
     // current point, whose Boyer-Lindquist position is `p`.
-    GARG_DEV f3 direction_bl(f3 p) const {
+    GARG_DEV f3 direction_bl(const KsConsts& k, f3 p) const {
         const KsState& v = e.d;
         const float r = e.r, r2 = r * r, q = e.q;
         const float d = r2 * r2 + k.a2 * s.z * s.z;
@@ -1112,37 +1170,88 @@ struct KerrSchildMarch {
         return len2 > 0.0f ? local * __builtin_amdgcn_rsqf(len2) : r_hat;
     }
 
+    // Continue a ray from Kerr-Schild state `state` whose azimuth equals
+    // the Boyer-Lindquist one at radius `r_phase` of its own phase: the
+    // drift references (Carter constant, axial angular momentum, scale), the
+    // drift so far, the step count and the gather carry over.
+    GARG_DEV void resume(float spin, float energy, const KsState& state, float launch_phase, float carter_ref,
+                         float l_ref, float scale_inv, const Gather& gathered, float drift_so_far,
+                         uint32_t steps_so_far) {
+        a = spin;
+        E = energy;
+        phase = launch_phase;
+        const KsConsts k = consts();
+        s = state;
+        c = KsState{0, 0, 0, 0, 0, 0};
+        e = ks_eval(k, s);
+        l0 = l_ref;
+        carter0 = carter_ref;
+        inv_scale = scale_inv;
+        inv_root_scale = sqrtf(scale_inv);
+        g = gathered;
+        prev = f3{s.x, s.y, s.z};
+        prev_r = e.r;
+        drift = drift_so_far;
+        steps = steps_so_far;
+        outcome = GARG_RAY_EXHAUSTED;
+        done = false;
+    }
+
     GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget) {
         run_loop<false>(kerr_loop(u, sc, gather, compensated), budget);
+    }
+
+    // The constants of this ray with the spin's taken from the loop's
+    // (scalar) parameters.
+    GARG_DEV KsConsts loop_consts(const KerrLoop& l) const {
+        return KsConsts{l.spin, l.a2, E, l.rp, l.rm, l.shift_scale, phase};
     }
 
     // Advance by at most `budget` steps (the contract of KerrFastMarch).
     template <bool UNIFORM, class Tick = NoTick>
     GARG_DEV void run_loop(KerrLoop l, uint32_t budget, Tick tick = Tick{}) {
         if constexpr (UNIFORM) l = wave_uniform(l);
-        const bool has_volume = l.gather != 0u;
-        const bool compensated = l.compensated != 0u;
+        const KsConsts k = loop_consts(l);
         for (uint32_t n = 0; n < budget && !done; ++n) {
             tick(n);
+            step(l, k);
+        }
+        if (!done && steps >= l.step_cap) done = true;
+    }
+
+    // One RK4 step with its drift gauges, path and gather; sets `done` at
+    // the cap, at capture and at escape.
+    GARG_DEV void step(const KerrLoop& l, const KsConsts& k) {
+        const bool has_volume = l.gather != 0u;
+        const bool compensated = l.compensated != 0u;
+        const float capture = k.rp + GARG_F(KERR_CAPTURE_MARGIN);
+        {
             if (steps >= l.step_cap) {
                 done = true;
-                break;
+                return;
             }
             const float r = e.r;
             // The Boyer-Lindquist step policy without its pole factor.
             const float h = fminf(fmaxf(fminf(l.step_factor * r, 0.2f * (r - k.rp)), GARG_F(KERR_STEP_FLOOR)),
                                   l.step_max);
             const float hh = 0.5f * h;
-            const KsState k2 = ks_eval(k, ks_axpy(s, e.d, hh)).d;
-            const KsState k3 = ks_eval(k, ks_axpy(s, k2, hh)).d;
-            const KsState k4 = ks_eval(k, ks_axpy(s, k3, h)).d;
+            // RK4, the increments summed as the stages produce them
+            // (e.d + 2 k2 + 2 k3 + k4), so no stage outlives the next.
+            KsState sum, kn;
+            kn = ks_eval(k, ks_axpy(s, e.d, hh)).d;
+            sum = ks_axpy(e.d, kn, 2.0f);
+            kn = ks_eval(k, ks_axpy(s, kn, hh)).d;
+            sum = ks_axpy(sum, kn, 2.0f);
+            kn = ks_eval(k, ks_axpy(s, kn, h)).d;
             const float w = h * (1.0f / 6.0f);
-            two_sum(s.x, c.x, ((e.d.x + k2.x * 2.0f) + (k3.x * 2.0f + k4.x)) * w);
-            two_sum(s.y, c.y, ((e.d.y + k2.y * 2.0f) + (k3.y * 2.0f + k4.y)) * w);
-            two_sum(s.z, c.z, ((e.d.z + k2.z * 2.0f) + (k3.z * 2.0f + k4.z)) * w);
-            two_sum(s.px, c.px, ((e.d.px + k2.px * 2.0f) + (k3.px * 2.0f + k4.px)) * w);
-            two_sum(s.py, c.py, ((e.d.py + k2.py * 2.0f) + (k3.py * 2.0f + k4.py)) * w);
-            two_sum(s.pz, c.pz, ((e.d.pz + k2.pz * 2.0f) + (k3.pz * 2.0f + k4.pz)) * w);
+            // Compensated always; without compensation the carries stay
+            // zero, which makes each update the plain s + increment.
+            fast_two_sum(s.x, c.x, sum.x + kn.x, w);
+            fast_two_sum(s.y, c.y, sum.y + kn.y, w);
+            fast_two_sum(s.z, c.z, sum.z + kn.z, w);
+            fast_two_sum(s.px, c.px, sum.px + kn.px, w);
+            fast_two_sum(s.py, c.py, sum.py + kn.py, w);
+            fast_two_sum(s.pz, c.pz, sum.pz + kn.pz, w);
             if (!compensated) c = KsState{0, 0, 0, 0, 0, 0};
             steps += 1;
             e = ks_eval(k, s);
@@ -1156,7 +1265,7 @@ struct KerrSchildMarch {
                 const float ct = s.z * e.ir;
                 const float sigma = r1 * r1 + k.a2 * (ct * ct);
                 float dn = fabsf(2.0f * sigma * hamilton) * inv_scale * fminf(e.ir * e.ir, 1.0f);
-                if (s.x * s.x + s.y * s.y > 1e-4f * e.q) dn = fmaxf(dn, fabsf(carter() - carter0) * inv_scale);
+                if (s.x * s.x + s.y * s.y > 1e-4f * e.q) dn = fmaxf(dn, fabsf(carter(k) - carter0) * inv_scale);
                 dn = fmaxf(dn, fabsf((s.x * s.py - s.y * s.px) - l0) * inv_root_scale);
                 drift = fmaxf(drift, dn);
             }
@@ -1184,13 +1293,13 @@ struct KerrSchildMarch {
                 if (r1 <= capture) {
                     outcome = GARG_RAY_CAPTURED;
                     done = true;
-                    break;
+                    return;
                 }
                 if (has_volume && fabsf(s.z) < GARG_F(DISK_SLAB_HALF_HEIGHT)) {
-                    const f3 p = position_bl();
+                    const f3 p = position_bl(k);
                     const float fbl = 1.0f - 2.0f * r1 * __builtin_amdgcn_rcpf(r1 * r1 + k.a2 * (s.z * e.ir) * (s.z * e.ir));
                     g.sample<true>(l.volume, l.opacity, l.redshift != 0u, p, path, fmaxf(fbl, 0.0f),
-                             [&]() { return direction_bl(p); });
+                             [&]() { return direction_bl(k, p); });
                 }
             }
             prev = f3{s.x, s.y, s.z};
@@ -1198,10 +1307,9 @@ struct KerrSchildMarch {
             if (r1 >= l.escape_radius) {
                 outcome = GARG_RAY_ESCAPED;
                 done = true;
-                break;
+                return;
             }
         }
-        if (!done && steps >= l.step_cap) done = true;
     }
 
     // Whether the ray falls into the hole: outward speed negative when the
@@ -1211,8 +1319,211 @@ struct KerrSchildMarch {
         if (outcome != GARG_RAY_EXHAUSTED) return false;
         return (s.x * e.d.x + s.y * e.d.y) * e.r * e.r + s.z * e.d.z * e.q < 0.0f;
     }
-    GARG_DEV f3 final_direction() const { return direction_bl(position_bl()); }
-    GARG_DEV f3 final_position() const { return position_bl(); }
+    GARG_DEV f3 final_direction() const {
+        const KsConsts k = consts();
+        return direction_bl(k, position_bl(k));
+    }
+    GARG_DEV f3 final_position() const { return position_bl(consts()); }
+};
+
+// ---------------------------------------------------------------------------
+// Kerr with Kerr-Schild coordinates only near the spin axis. Boyer-Lindquist
+// coordinates are singular on the axis, and every ray whose Boyer-Lindquist
+// march breaks the drift budget passes close to it (on the metric's 1080p
+// frame at a = 0.9 every such ray dips below sin(theta) = 0.2); away from it
+// the fast Boyer-Lindquist step is the cheaper one. A ray therefore marches
+// in the fast Boyer-Lindquist form until sin(theta) first falls below
+// GARG_POLAR_ENTER and in Kerr-Schild coordinates from there to its end.
+// Both forms take the same step length there (the pole factor is 1 above
+// sin(theta) = 0.08); the launch's Carter constant stays the drift
+// reference and the gather carries across; the coordinate change runs in
+// double precision, once per polar ray.
+#define GARG_POLAR_ENTER 0.25f
+
+// Boyer-Lindquist (r, theta, phi, p_r, p_theta; E, L) to Kerr-Schild
+// (x, y, z, p_x, p_y, p_z), with the Kerr-Schild azimuth equal to phi here:
+// ks_from_bl in double precision.
+GARG_DEV KsState ks_from_bl_f64(double a, const KerrRayState& st) {
+    const double a2 = a * a, r = st.r;
+    double sth, cth, sph, cph;
+    sincos((double)st.theta, &sth, &cth);
+    sincos((double)st.phi, &sph, &cph);
+    const double sin2 = fmax(sth * sth, 1e-24);
+    const double delta = r * r - 2.0 * r + a2, sigma = r * r + a2 * cth * cth;
+    const double E = st.energy, L = st.angular_momentum;
+    const double g_tt = -((r * r + a2) * (r * r + a2) - a2 * delta * sin2) / (sigma * delta);
+    const double g_tphi = -2.0 * a * r / (sigma * delta);
+    const double g_phiphi = (delta - a2 * sin2) / (sigma * delta * sin2);
+    const double kt = g_tt * (-E) + g_tphi * L;
+    const double kphi = g_tphi * (-E) + g_phiphi * L;
+    const double kr = delta * (double)st.pr / sigma, kth = (double)st.ptheta / sigma;
+    const double kt_ks = kt + 2.0 * r / delta * kr;
+    const double kphi_ks = kphi + a / delta * kr;
+    const double x = (r * cph - a * sph) * sth, y = (r * sph + a * cph) * sth, z = r * cth;
+    const double vx = cph * sth * kr + (r * cph - a * sph) * cth * kth - y * kphi_ks;
+    const double vy = sph * sth * kr + (r * sph + a * cph) * cth * kth + x * kphi_ks;
+    const double vz = cth * kr - r * sth * kth;
+    const double q = r * r + a2;
+    const double f = 2.0 * r * r * r / (r * r * r * r + a2 * z * z);
+    const double lx = (r * x + a * y) / q, ly = (r * y - a * x) / q, lz = z / r;
+    const double lk = kt_ks + (lx * vx + ly * vy) + lz * vz;
+    return KsState{(float)x, (float)y, (float)z, (float)(vx + f * lx * lk), (float)(vy + f * ly * lk),
+                   (float)(vz + f * lz * lk)};
+}
+
+// The ray is kept in a KerrSchildMarch. While it is in the Boyer-Lindquist
+// form that struct's float fields hold the KerrFastMarch fields instead
+// (unpack/pack below, plain value copies the compiler keeps in the same
+// registers): the shared fields (gather, drift, step count, outcome, done,
+// Carter reference, scale) in their own places, E in E, the launch L in l0,
+// the state, carries and evaluation in the Kerr-Schild state, carries and
+// evaluation, the previous point in prev, and the products of E and L in
+// the remaining floats. The spin's constants come from the loop's scalar
+// parameters.
+struct KerrHybridMarch {
+    KerrSchildMarch ks;
+    bool polar;  // the Kerr-Schild form carries the ray
+
+    GARG_DEV KerrFastMarch unpack(float a, float a2, float rplus) const {
+        KerrFastMarch m;
+        m.k.a = a;
+        m.k.a2 = a2;
+        m.k.E = ks.E;
+        m.k.L = ks.l0;
+        m.k.aE = ks.phase;
+        m.k.aL = ks.inv_root_scale;
+        m.k.L2 = ks.prev_r;
+        m.k.a2E2 = ks.s.pz;
+        m.k.two_aEL = ks.c.pz;
+        m.k.rplus = rplus;
+        m.r = ks.s.x;
+        m.th = ks.s.y;
+        m.ph = ks.s.z;
+        m.pr = ks.s.px;
+        m.pth = ks.s.py;
+        m.cr = ks.c.x;
+        m.cth = ks.c.y;
+        m.cph = ks.c.z;
+        m.cpr = ks.c.px;
+        m.cpth = ks.c.py;
+        m.e.dr = ks.e.d.x;
+        m.e.dth = ks.e.d.y;
+        m.e.dph = ks.e.d.z;
+        m.e.dpr = ks.e.d.px;
+        m.e.dpth = ks.e.d.py;
+        m.e.st = ks.e.d.pz;
+        m.e.ct = ks.e.r;
+        m.e.null_residual = ks.e.ir;
+        m.e.carter = ks.e.q;
+        m.e.inv_sigma = ks.e.f;
+        m.e.inv_r2 = ks.e.big;
+        m.carter0 = ks.carter0;
+        m.inv_scale = ks.inv_scale;
+        m.capture = rplus + GARG_F(KERR_CAPTURE_MARGIN);
+        m.g = ks.g;
+        m.prev_rho = ks.prev.x;
+        m.prev_z = ks.prev.y;
+        m.prev_ph = ks.prev.z;
+        m.drift = ks.drift;
+        m.steps = ks.steps;
+        m.outcome = ks.outcome;
+        m.done = ks.done;
+        return m;
+    }
+
+    GARG_DEV void pack(const KerrFastMarch& m) {
+        ks.a = m.k.a;
+        ks.E = m.k.E;
+        ks.l0 = m.k.L;
+        ks.phase = m.k.aE;
+        ks.inv_root_scale = m.k.aL;
+        ks.prev_r = m.k.L2;
+        ks.s = KsState{m.r, m.th, m.ph, m.pr, m.pth, m.k.a2E2};
+        ks.c = KsState{m.cr, m.cth, m.cph, m.cpr, m.cpth, m.k.two_aEL};
+        ks.e.d = KsState{m.e.dr, m.e.dth, m.e.dph, m.e.dpr, m.e.dpth, m.e.st};
+        ks.e.r = m.e.ct;
+        ks.e.ir = m.e.null_residual;
+        ks.e.q = m.e.carter;
+        ks.e.f = m.e.inv_sigma;
+        ks.e.big = m.e.inv_r2;
+        ks.carter0 = m.carter0;
+        ks.inv_scale = m.inv_scale;
+        ks.g = m.g;
+        ks.prev = f3{m.prev_rho, m.prev_z, m.prev_ph};
+        ks.drift = m.drift;
+        ks.steps = m.steps;
+        ks.outcome = m.outcome;
+        ks.done = m.done;
+    }
+
+    GARG_DEV KerrFastMarch unpack() const {
+        const float a = ks.a, a2 = a * a;
+        return unpack(a, a2, kerr_outer_horizon(a));
+    }
+
+    // From the Boyer-Lindquist form to the Kerr-Schild form at the current point.
+    GARG_DEV void to_polar(const KerrFastMarch& m) {
+        const KerrRayState st{m.r, m.th, m.ph, m.pr, m.pth, m.k.E, m.k.L};
+        const KsState state = ks_from_bl_f64(m.k.a, st);
+        const double root = sqrt(fmax(1.0 - (double)m.k.a * m.k.a, 0.0));
+        const double ratio = fabs(((double)st.r - (1.0 + root)) / ((double)st.r - (1.0 - root)));
+        ks.resume(m.k.a, st.energy, state, (float)log(ratio), m.carter0, m.k.L, m.inv_scale, m.g, m.drift, m.steps);
+        polar = true;
+    }
+
+    GARG_DEV void begin(float spin, const KerrRayState& launched) {
+        KerrFastMarch m;
+        m.begin(spin, launched);
+        polar = false;
+        pack(m);
+        if (m.e.st < GARG_POLAR_ENTER) to_polar(m);
+    }
+
+    GARG_DEV bool done() const { return ks.done; }
+
+    GARG_DEV void run(const garg_march_uniforms& u, const Scene& sc, bool gather, bool compensated, uint32_t budget) {
+        run_loop<false>(kerr_loop(u, sc, gather, compensated), budget);
+    }
+
+    // Two phases, so that only one form's state is live at a time: the
+    // Boyer-Lindquist steps until the ray ends or reaches the axis, then the
+    // Kerr-Schild steps for the rest of the budget.
+    template <bool UNIFORM, class Tick = NoTick>
+    GARG_DEV void run_loop(KerrLoop l, uint32_t budget, Tick tick = Tick{}) {
+        if constexpr (UNIFORM) l = wave_uniform(l);
+        uint32_t n = 0;
+        if (!polar) {
+            KerrFastMarch m = unpack(l.spin, l.a2, l.rp);
+            bool axis = false;
+            for (; n < budget && !m.done; ++n) {
+                tick(n);
+                m.step(l);
+                if (!m.done && m.e.st < GARG_POLAR_ENTER) {
+                    axis = true;
+                    ++n;
+                    break;
+                }
+            }
+            if (axis) to_polar(m);
+            else pack(m);
+        }
+        if (polar) {
+            const KsConsts k = ks.loop_consts(l);
+            for (; n < budget && !ks.done; ++n) {
+                tick(n);
+                ks.step(l, k);
+            }
+        }
+        if (!ks.done && ks.steps >= l.step_cap) ks.done = true;
+    }
+
+    GARG_DEV const Gather& gather() const { return ks.g; }
+    GARG_DEV float max_drift() const { return ks.drift; }
+    GARG_DEV uint32_t step_count() const { return ks.steps; }
+    GARG_DEV uint32_t result() const { return ks.outcome; }
+    GARG_DEV bool falls_in() const { return polar ? ks.falls_in() : unpack().falls_in(); }
+    GARG_DEV f3 final_direction() const { return polar ? ks.final_direction() : unpack().final_direction(); }
+    GARG_DEV f3 final_position() const { return polar ? ks.final_position() : unpack().final_position(); }
 };
 
 // Colour of a finished ray: gas plus the starfield behind it unless it falls

@@ -53,8 +53,8 @@ struct BatchRequest {
 struct BatchArgs {
     const BatchRequest* requests;   // sorted by queue_begin, first at 0
     uint32_t request_count;
-    uint32_t kind;                  // BATCH_SCHWARZSCHILD (every request has spin 0), BATCH_KERR or
-                                    // BATCH_KERR_SCHILD (every request has spin > 0)
+    uint32_t kind;                  // BATCH_SCHWARZSCHILD (every request has spin 0), BATCH_KERR,
+                                    // BATCH_KERR_SCHILD or BATCH_KERR_POLAR (every request has spin > 0)
     uint64_t total_rays;
     unsigned long long* queue;      // refill policy: eight {head, end} pairs, 128
                                     // bytes apart; tile policy: one head
@@ -67,7 +67,7 @@ struct BatchArgs {
 };
 
 // The batched march's ray state machines.
-enum { BATCH_SCHWARZSCHILD = 0, BATCH_KERR = 1, BATCH_KERR_SCHILD = 2 };
+enum { BATCH_SCHWARZSCHILD = 0, BATCH_KERR = 1, BATCH_KERR_SCHILD = 2, BATCH_KERR_POLAR = 3 };
 
 hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block, hipStream_t stream);
 uint32_t march_batch_resident_blocks(uint32_t kind, bool aging, bool tiles, uint32_t block);

@@ -42,6 +42,15 @@ GARG_DEV PixelResult shade_pixel(const garg_march_uniforms& u, const Scene& sc, 
             out.steps = m.steps;
             out.outcome = m.outcome;
             out.depth = fminf(m.g.depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f);
+        } else if (Form == GARG_INTEGRATOR_POLAR) {
+            KerrHybridMarch m;
+            m.begin(u.spin, launched);
+            m.run(u, sc, true, u.compensated != 0u, cap);
+            out.color = finish_color(m.gather(), m.falls_in(), m.final_direction(), u, sc);
+            out.drift = m.max_drift();
+            out.steps = m.step_count();
+            out.outcome = m.result();
+            out.depth = fminf(m.gather().depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f);
         } else {
             KerrFastMarch m;
             m.begin(u.spin, launched);
@@ -125,6 +134,15 @@ __global__ __launch_bounds__(64) void probe_kernel(ProbeArgs a) {
             out.drift = m.drift;
             out.outcome = m.outcome;
             out.steps = m.steps;
+        } else if (Form == GARG_INTEGRATOR_POLAR) {
+            KerrHybridMarch m;
+            m.begin(u.spin, launched);
+            m.run(u, sc, false, u.compensated != 0u, u.geodesic.step_cap);
+            pos = m.final_position();
+            d = m.final_direction();
+            out.drift = m.max_drift();
+            out.outcome = m.result();
+            out.steps = m.step_count();
         } else {
             KerrFastMarch m;
             m.begin(u.spin, launched);
@@ -175,6 +193,8 @@ hipError_t launch_march_image(const MarchImageArgs& a, uint32_t integrator, hipS
         march_image_kernel<GARG_INTEGRATOR_REFERENCE><<<grid, 64, 0, stream>>>(a);
     else if (integrator == GARG_INTEGRATOR_KERR_SCHILD)
         march_image_kernel<GARG_INTEGRATOR_KERR_SCHILD><<<grid, 64, 0, stream>>>(a);
+    else if (integrator == GARG_INTEGRATOR_POLAR)
+        march_image_kernel<GARG_INTEGRATOR_POLAR><<<grid, 64, 0, stream>>>(a);
     else
         march_image_kernel<GARG_INTEGRATOR_FAST><<<grid, 64, 0, stream>>>(a);
     return hipGetLastError();
@@ -186,6 +206,8 @@ hipError_t launch_probe(const ProbeArgs& a, uint32_t integrator, hipStream_t str
         probe_kernel<GARG_INTEGRATOR_REFERENCE><<<grid, 64, 0, stream>>>(a);
     else if (integrator == GARG_INTEGRATOR_KERR_SCHILD)
         probe_kernel<GARG_INTEGRATOR_KERR_SCHILD><<<grid, 64, 0, stream>>>(a);
+    else if (integrator == GARG_INTEGRATOR_POLAR)
+        probe_kernel<GARG_INTEGRATOR_POLAR><<<grid, 64, 0, stream>>>(a);
     else
         probe_kernel<GARG_INTEGRATOR_FAST><<<grid, 64, 0, stream>>>(a);
     return hipGetLastError();

@@ -139,6 +139,27 @@ template <>
 struct Lane<BATCH_KERR_SCHILD> : KerrLane<KerrSchildMarch> {};
 
 template <>
+struct Lane<BATCH_KERR_POLAR> {
+    KerrHybridMarch m;
+    GARG_DEV bool start(const BatchRequest& r, uint32_t px, uint32_t py, CounterTable&, uint32_t, uint32_t) {
+        const garg_march_uniforms& u = r.uniforms;
+        f3 dir = camera_direction(u, px, py);
+        m.begin(u.spin, kerr_launch(u.spin, load3(u.camera_position), dir));
+        return true;
+    }
+    template <bool UNIFORM, class Tick = NoTick>
+    GARG_DEV void run(const BatchRequest& r, uint32_t budget, Tick tick = Tick{}) {
+        m.template run_loop<UNIFORM>(kerr_loop(r.uniforms, r.scene, true, r.uniforms.compensated != 0u), budget, tick);
+    }
+    GARG_DEV bool done() const { return m.done(); }
+    GARG_DEV void finish(const BatchRequest& r, CounterTable& t, uint32_t request, uint32_t pixel) {
+        f3 color = finish_color(m.gather(), m.falls_in(), m.final_direction(), r.uniforms, r.scene);
+        finish_ray(r, t, request, pixel, color, m.max_drift(), m.step_count(), m.result(),
+                   fminf(m.gather().depth() / (2.0f * GARG_F(R_ESCAPE)), 1.0f));
+    }
+};
+
+template <>
 struct Lane<BATCH_SCHWARZSCHILD> {
     SchwMarchT<true> m;
     // Rays that never enter the integration sphere finish at the jump.
@@ -270,7 +291,8 @@ GARG_DEV bool march_lanes(const BatchArgs& a, Lane<KIND>& ray, const LaneRay& id
 // Kerr-Schild rays carry more state than 128 registers hold next to the
 // refill bookkeeping, so that kind runs three waves per SIMD here.
 template <int KIND, int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(KIND == BATCH_KERR_SCHILD ? 3 : 4))) void
+__global__ __launch_bounds__(BLOCK) __attribute__((
+    amdgpu_waves_per_eu(KIND == BATCH_KERR_SCHILD || KIND == BATCH_KERR_POLAR ? 3 : 4))) void
 march_batch_kernel(BatchArgs a) {
     __shared__ uint32_t s_counts[GARG_BATCH_MAX_REQUESTS * 4];
     __shared__ unsigned long long s_steps[GARG_BATCH_MAX_REQUESTS];
@@ -430,6 +452,7 @@ uint32_t march_batch_resident_blocks(uint32_t kind, bool aging, bool tiles, uint
                          : resident_blocks<BATCH_SCHWARZSCHILD, false>(tiles, block);
         case BATCH_KERR: return resident_blocks<BATCH_KERR, true>(tiles, block);
         case BATCH_KERR_SCHILD: return resident_blocks<BATCH_KERR_SCHILD, true>(tiles, block);
+        case BATCH_KERR_POLAR: return resident_blocks<BATCH_KERR_POLAR, true>(tiles, block);
         default: return 0u;
     }
 }
@@ -445,6 +468,7 @@ hipError_t launch_march_batch(const BatchArgs& a, uint32_t grid, uint32_t block,
                            : launch_block<BATCH_SCHWARZSCHILD, false>(a, grid, block, stream);
         case BATCH_KERR: return launch_block<BATCH_KERR, true>(a, grid, block, stream);
         case BATCH_KERR_SCHILD: return launch_block<BATCH_KERR_SCHILD, true>(a, grid, block, stream);
+        case BATCH_KERR_POLAR: return launch_block<BATCH_KERR_POLAR, true>(a, grid, block, stream);
         default: return hipErrorInvalidValue;
     }
 }

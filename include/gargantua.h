@@ -56,9 +56,19 @@ enum { GARG_GENERATOR_DISK = 0, GARG_GENERATOR_ANNULUS = 1 };
  * future work for its polar seam (docs/lab/07-kerr.md:33-45). It takes the
  * same launch, step policy (without the pole factor), gas and sky as the
  * Boyer-Lindquist forms; at spin 0 it is the FAST Schwarzschild form.
+ * POLAR marches a Kerr ray in the FAST form while sin(theta) >= 0.25 and in
+ * the KERR_SCHILD form from there until sin(theta) > 0.3, converting the
+ * state in double precision at each switch: regular on the axis at about the
+ * cost of the FAST form, since only rays that pass near the axis leave it;
+ * at spin 0 it is the FAST Schwarzschild form.
  * The batched march runs the FAST form, or for spinning requests the
- * Kerr-Schild form when garg_batch_config.kerr_schild asks for it. */
-enum { GARG_INTEGRATOR_REFERENCE = 0, GARG_INTEGRATOR_FAST = 1, GARG_INTEGRATOR_KERR_SCHILD = 2 };
+ * KERR_SCHILD or POLAR form when garg_batch_config.kerr_schild asks for it. */
+enum {
+    GARG_INTEGRATOR_REFERENCE = 0,
+    GARG_INTEGRATOR_FAST = 1,
+    GARG_INTEGRATOR_KERR_SCHILD = 2,
+    GARG_INTEGRATOR_POLAR = 3
+};
 
 typedef struct garg_ctx garg_ctx;
 typedef struct garg_stream garg_stream;
@@ -301,7 +311,8 @@ typedef struct {
                                    live; 0 = whole-tile claims: a wave marches 64 rays to
                                    the end before it takes the next 64 */
     uint32_t kerr_schild;       /* spinning requests: 0 = the Boyer-Lindquist FAST form,
-                                   1 = the Kerr-Schild form (regular on the spin axis) */
+                                   1 = the Kerr-Schild form (regular on the spin axis),
+                                   2 = the POLAR form (Kerr-Schild near the axis only) */
 } garg_batch_config;
 
 /* Rays of a request in tile order: 8x8 pixel tiles row-major, 64 rays each. */

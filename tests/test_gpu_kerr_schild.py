@@ -1,9 +1,11 @@
-"""GPU parity of the Kerr-Schild integrator form (GARG_INTEGRATOR_KERR_SCHILD)
-against the FP64 Kerr-Schild oracle (oracle/kerr_schild.cpp), which is
-pinned to the Boyer-Lindquist metric and integrator of KerrGeodesic.swift:
-probes over the metric's view including the column over the pole, the drift
-budget on the whole image, and spin zero falling back to the Schwarzschild
-fast form."""
+"""GPU parity of the forms regular on the spin axis: the Kerr-Schild form
+(GARG_INTEGRATOR_KERR_SCHILD) and the polar form (GARG_INTEGRATOR_POLAR:
+Boyer-Lindquist fast steps until the ray first comes within sin(theta) <
+0.25 of the axis, Kerr-Schild steps from there), against the FP64
+Kerr-Schild oracle (oracle/kerr_schild.cpp), which is pinned to the
+Boyer-Lindquist metric and integrator of KerrGeodesic.swift: probes over the
+metric's view including the column over the pole, the drift budget on the
+whole image, and spin zero falling back to the Schwarzschild fast form."""
 import ctypes as C
 import math
 
@@ -18,6 +20,8 @@ from tests.test_gpu_march import same_bits
 pytestmark = pytest.mark.gpu
 
 SPIN = 0.9
+REGULAR = pytest.mark.parametrize("form", [api.INTEGRATOR_KERR_SCHILD, api.INTEGRATOR_POLAR],
+                                  ids=["kerr-schild", "polar"])
 
 
 def view(cols, rows, kind="still", cap=4096, spin=SPIN):
@@ -45,9 +49,10 @@ def angle(x, y):
     return math.atan2(np.linalg.norm(np.cross(x, y)), float(x @ y))
 
 
+@REGULAR
 @pytest.mark.parametrize("spin,kind,budget", [(0.9, "still", 1e-5), (0.5, "still", 1e-5), (0.99, "still", 1e-5),
                                                (0.9, "interactive", 1e-4)])
-def test_kerr_schild_probes_match_the_fp64_oracle(ctx, spin, kind, budget):
+def test_kerr_schild_probes_match_the_fp64_oracle(ctx, spin, kind, budget, form):
     """A 16 x 9 grid of the metric's view (r = 50, inclination 60 deg) plus
     the frame's centre column, which passes over the pole, at spins 0.5,
     0.9 and 0.99 (horizons 1.87, 1.44, 1.14) with the still policy and at
@@ -56,11 +61,13 @@ def test_kerr_schild_probes_match_the_fp64_oracle(ctx, spin, kind, budget):
     edge), step counts within 1%, escape directions within 2e-5 rad (FP32
     against FP64 at the same steps; compensated for stills), and every
     escaping ray inside the drift budget of its policy (1e-5 still, 1e-4
-    interactive)."""
+    interactive). The polar form integrates the Boyer-Lindquist equations
+    with the same steps away from the axis, where both converge to the same
+    geodesics, and is held to the same bar."""
     cam, g, u, grid = view(16, 9, kind, 4096 if kind == "still" else 256, spin)
     column = [p for p in scenes.camera_grid(cam, 33, 33, 16.0 / 9.0)[16::33]]
     launches = grid + column
-    res = ctx.probe(launches, u, integrator=api.INTEGRATOR_KERR_SCHILD)
+    res = ctx.probe(launches, u, integrator=form)
     worst_angle, worst_drift, mismatches, compared = 0.0, 0.0, 0, 0
     for (o, d), r in zip(launches, res):
         ref, dref = oracle_ks(o, d, g, spin)
@@ -72,13 +79,14 @@ def test_kerr_schild_probes_match_the_fp64_oracle(ctx, spin, kind, budget):
             worst_angle = max(worst_angle, angle(r.direction[:], dref))
             worst_drift = max(worst_drift, r.drift)
             compared += 1
-    print(f"kerr-schild probes a={spin} {kind}: {compared} escaped compared, {mismatches} fate mismatches, "
+    print(f"form {form} probes a={spin} {kind}: {compared} escaped compared, {mismatches} fate mismatches, "
           f"max angle {worst_angle:.2e} rad, max drift {worst_drift:.2e}")
     assert compared > 0.5 * len(launches) and mismatches <= 2
     assert worst_angle < 2e-5 and worst_drift < budget
 
 
-def test_kerr_schild_fixes_the_polar_column(ctx):
+@REGULAR
+def test_kerr_schild_fixes_the_polar_column(ctx, form):
     """Rays of the centre column pass near the pole, where the
     Boyer-Lindquist forms clamp 1/sin^2 theta and shrink their steps: on
     them the Kerr-Schild form lands closer to the converged FP64 direction
@@ -87,7 +95,7 @@ def test_kerr_schild_fixes_the_polar_column(ctx):
     1e-5 rad, and keeps its drift inside the budget on all of them."""
     cam, g, u, _ = view(1, 1)
     column = scenes.camera_grid(cam, 1, 41, 16.0 / 9.0)
-    ks = ctx.probe(column, u, integrator=api.INTEGRATOR_KERR_SCHILD)
+    ks = ctx.probe(column, u, integrator=form)
     bl = ctx.probe(column, u, integrator=api.INTEGRATOR_FAST)
     fine = api.settings("still", step_cap=400_000, escape_radius=60.0, step_factor=0.001)
     worse, compared = 0, 0
@@ -109,13 +117,14 @@ def test_kerr_schild_image_keeps_the_drift_budget(ctx):
     """The metric's view at 480 x 270 with the 4096-step still budget: the
     Boyer-Lindquist forms take a few percent of the escaping rays over the
     1e-5 drift budget (near-polar rays, in FP64 too); the Kerr-Schild form,
-    whose metric is regular on the axis, keeps at most 0.1% over it."""
+    whose metric is regular on the axis, and the polar form, which switches
+    to it near the axis, keep at most 0.1% over it."""
     w, h = 480, 270
     cam = scenes.look_at(50.0, 60.0, 60.0)
     u = api.uniforms(cam, w, h, api.settings("still", escape_radius=60.0), spin=SPIN, compensated=True,
                      star_seed=1, drift_budget=1e-5)
     shares = {}
-    for form in (api.INTEGRATOR_FAST, api.INTEGRATOR_KERR_SCHILD):
+    for form in (api.INTEGRATOR_FAST, api.INTEGRATOR_KERR_SCHILD, api.INTEGRATOR_POLAR):
         t = api.Targets(ctx, w, h)
         t.reset_counters()
         ctx.march_image(u, t, None, None, form)
@@ -124,12 +133,14 @@ def test_kerr_schild_image_keeps_the_drift_budget(ctx):
         kept = debug[..., 2] != api.RAY_CAPTURED
         shares[form] = float((debug[..., 0][kept] > 1e-5).mean())
     print(f"over the drift budget: Boyer-Lindquist fast {shares[api.INTEGRATOR_FAST]:.2%}, "
-          f"Kerr-Schild {shares[api.INTEGRATOR_KERR_SCHILD]:.2%}")
-    assert shares[api.INTEGRATOR_KERR_SCHILD] <= 0.001
-    assert shares[api.INTEGRATOR_FAST] > 10 * max(shares[api.INTEGRATOR_KERR_SCHILD], 1e-4)
+          f"Kerr-Schild {shares[api.INTEGRATOR_KERR_SCHILD]:.2%}, polar {shares[api.INTEGRATOR_POLAR]:.2%}")
+    for form in (api.INTEGRATOR_KERR_SCHILD, api.INTEGRATOR_POLAR):
+        assert shares[form] <= 0.001
+        assert shares[api.INTEGRATOR_FAST] > 10 * max(shares[form], 1e-4)
 
 
-def test_kerr_schild_image_agrees_with_boyer_lindquist_off_the_axis(ctx):
+@REGULAR
+def test_kerr_schild_image_agrees_with_boyer_lindquist_off_the_axis(ctx, form):
     """With the gas: where the two coordinate systems both integrate the
     same geodesics accurately (rays that stay away from the pole), the
     pictures agree. Compared outside the middle quarter of the columns: the
@@ -145,12 +156,12 @@ def test_kerr_schild_image_agrees_with_boyer_lindquist_off_the_axis(ctx):
     u = api.uniforms(cam, w, h, api.settings("still", escape_radius=60.0), spin=SPIN, compensated=True,
                      star_seed=1)
     out = {}
-    for form in (api.INTEGRATOR_FAST, api.INTEGRATOR_KERR_SCHILD):
+    for f in (api.INTEGRATOR_FAST, form):
         t = api.Targets(ctx, w, h)
-        ctx.march_image(u, t, vol, None, form)
-        out[form] = t.read()
+        ctx.march_image(u, t, vol, None, f)
+        out[f] = t.read()
         t.close()
-    (ca, da, _), (cb, db, _) = out[api.INTEGRATOR_FAST], out[api.INTEGRATOR_KERR_SCHILD]
+    (ca, da, _), (cb, db, _) = out[api.INTEGRATOR_FAST], out[form]
     side = np.abs(np.arange(w) + 0.5 - 0.5 * w) > w / 8
     fate = da[:, side, 2] == db[:, side, 2]
     peak = float(np.abs(ca[..., :3]).max())
@@ -161,22 +172,24 @@ def test_kerr_schild_image_agrees_with_boyer_lindquist_off_the_axis(ctx):
     assert fate.mean() >= 0.995 and close >= 0.99
 
 
-def test_kerr_schild_at_zero_spin_is_the_schwarzschild_fast_form(ctx):
+@REGULAR
+def test_kerr_schild_at_zero_spin_is_the_schwarzschild_fast_form(ctx, form):
     from tests.test_gpu_march import disk_volume
     vol = disk_volume(ctx)
     cam = api.orbit_camera()
     u = api.uniforms(cam, 96, 64, api.settings("still"), compensated=True, star_seed=1)
     images = []
-    for form in (api.INTEGRATOR_FAST, api.INTEGRATOR_KERR_SCHILD):
+    for f in (api.INTEGRATOR_FAST, form):
         t = api.Targets(ctx, 96, 64)
-        ctx.march_image(u, t, vol, None, form)
+        ctx.march_image(u, t, vol, None, f)
         images.append(t.read())
         t.close()
     assert same_bits(images[0], images[1])
 
 
+@REGULAR
 @pytest.mark.parametrize("threshold", [0, 40])
-def test_batched_kerr_schild_matches_plain_bitwise(ctx, threshold):
+def test_batched_kerr_schild_matches_plain_bitwise(ctx, threshold, form):
     """The batched persistent march with garg_batch_config.kerr_schild runs
     the same per-ray state machine as the plain Kerr-Schild launch: every
     target bit-identical, for whole-tile claims and for lane refill, alone
@@ -185,9 +198,10 @@ def test_batched_kerr_schild_matches_plain_bitwise(ctx, threshold):
     vol = disk_volume(ctx)
     a = kerr_request(ctx, 256, 144, 0.9, 60.0, 50.0, 4096, vol)
     b = kerr_request(ctx, 120, 72, 0.5, 80.0, 40.0, 256, vol, compensated=False)
-    plain_a = render_plain(ctx, a, api.INTEGRATOR_KERR_SCHILD)[0]
-    plain_b = render_plain(ctx, b, api.INTEGRATOR_KERR_SCHILD)[0]
-    cfg = ctx.batch_config(compaction_period=8, refill_threshold=threshold, kerr_schild=1)
+    plain_a = render_plain(ctx, a, form)[0]
+    plain_b = render_plain(ctx, b, form)[0]
+    cfg = ctx.batch_config(compaction_period=8, refill_threshold=threshold,
+                           kerr_schild=1 if form == api.INTEGRATOR_KERR_SCHILD else 2)
     ctx.march_batch([a], cfg)
     assert same_bits(plain_a, a["targets"].read())
     ctx.march_batch([b, a], cfg)

@@ -211,8 +211,8 @@ def _image_vs_oracle(ctx, req, field, integrator, tables=None, samples=400, seed
     return outcome_ok / samples, steps_close / samples, color_ok / samples
 
 
-@pytest.mark.parametrize("integrator", [api.INTEGRATOR_REFERENCE, api.INTEGRATOR_FAST, api.INTEGRATOR_KERR_SCHILD],
-                         ids=["reference", "fast", "kerr-schild"])
+@pytest.mark.parametrize("integrator", [api.INTEGRATOR_REFERENCE, api.INTEGRATOR_FAST, api.INTEGRATOR_KERR_SCHILD,
+                                        api.INTEGRATOR_POLAR], ids=["reference", "fast", "kerr-schild", "polar"])
 def test_kerr_image_matches_oracle_per_pixel(ctx, integrator):
     """400 random pixels of a 160 x 90 frame with gas against the FP64
     Boyer-Lindquist march: outcomes, step counts and colours. The
