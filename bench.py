@@ -3,7 +3,11 @@
 march at 1920x1080, spin 0.9, 4096-step budget, observer at r = 50,
 inclination 60 degrees, 60 degree field of view, through the 1M-particle /
 128^3 disk volume (settled 600 substeps), against the plain
-one-launch-per-frame march on the same GPU in the same run.
+one-launch-per-frame march on the same GPU in the same run. The headline
+form is the one that keeps every escaping ray's drift under the 1e-5 still
+budget: the polar form (Boyer-Lindquist fast steps away from the spin axis,
+Kerr-Schild steps near it); the Boyer-Lindquist fast form and the all
+Kerr-Schild form are side legs.
 
 One bench step is one frame: every ray of the 1080p image marched once.
 Inputs (particles, volume, tables) are resident in HBM before timing; the
@@ -76,6 +80,11 @@ def build(ctx, api):
     return p, vol, u, targets
 
 
+def progress(text):
+    """One line per finished leg on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {text}", file=sys.stderr, flush=True)
+
+
 def time_frames(ctx, api, run, warmup, steps):
     for _ in range(warmup):
         run()
@@ -95,12 +104,24 @@ def time_frames(ctx, api, run, warmup, steps):
     return total_ms, frames
 
 
+def host_threads():
+    """Cores this process may use: those it is allowed to run on, within the
+    thread count the host grants it (OMP_NUM_THREADS: a GPU box gives one
+    GPU's process a share of 16 cores)."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(allowed, share) if share > 0 else allowed), allowed
+
+
 def cpu_baseline(u, vol, every: int):
     """The FP64 oracle's Kerr march (march.metal:123-184 restated) on every
     `every`-th row of the same rays and volume, on the host's cores."""
     import oracle as O
     from tests import scenes
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, allowed = host_threads()
     field = scenes.field_from_volume(vol)
     p = scenes.oracle_params(u)
     w, h = int(u.resolution[0]), int(u.resolution[1])
@@ -113,7 +134,8 @@ def cpu_baseline(u, vol, every: int):
     return {"value": total / dt, "unit": "geodesic steps/s", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle Kerr march with gas on every {every}th row of the 1920x1080 frame "
                       f"({len(xy)} rays, {total} steps, {dt:.1f} s)",
-            "cpu": _cpu_model()}
+            "cpu": _cpu_model(), "nproc": os.cpu_count(), "cores_allowed": allowed,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def _cpu_model():
@@ -137,92 +159,90 @@ def main():
     p, vol, u, targets = build(ctx, api)
 
     from gargantua_amd import autotune
-    cfg = autotune.best(ctx, "headline")
+    base = autotune.best(ctx, "headline")
     if args.config:
         b, w, per, thr = (int(x) for x in args.config.split(","))
-        cfg = ctx.batch_config(block=b, waves_per_simd=w, compaction_period=per, refill_threshold=thr)
+        base = ctx.batch_config(block=b, waves_per_simd=w, compaction_period=per, refill_threshold=thr)
+
+    def form_config(kerr_schild):
+        return ctx.batch_config(block=base.block, waves_per_simd=base.waves_per_simd,
+                                compaction_period=base.compaction_period, refill_threshold=base.refill_threshold,
+                                kerr_schild=kerr_schild)
+
+    cfg = form_config(2)  # the polar form
     request = [{"uniforms": u, "targets": targets, "volume": vol}]
 
-    # Steps per frame are a property of the scene: count them once, with
-    # the frame's drift (maximum over each ray's steps of the Carter and
-    # null-residual drifts) against the still budget: over the rays that
-    # escape or run out of budget, and the counter's figure that includes
-    # captured rays (whose gauges diverge at the horizon).
+    def frame_drift(t, counters):
+        """The frame's drift (maximum over each ray's steps of the Carter,
+        null-residual and axial-momentum drifts) against the still budget:
+        over the rays that escape or run out of budget, and the counter's
+        figure that includes captured rays (whose gauges diverge at the
+        horizon)."""
+        debug = t.read()[1]
+        kept = debug[..., 2] != api.RAY_CAPTURED
+        return {"budget": float(u.drift_budget), "max_escaped_or_exhausted": float(debug[..., 0][kept].max()),
+                "rays_over_budget_escaped_or_exhausted": int((debug[..., 0][kept] > u.drift_budget).sum()),
+                "rays_over_budget_all": int(counters.over_drift_budget), "captured": int(counters.captured),
+                "exhausted": int(counters.exhausted)}
+
+    # Steps per frame are a property of the scene and form: count them once.
     targets.reset_counters()
     totals = ctx.march_batch(request, cfg)
     steps_per_frame = int(totals.steps)
     rays = int(totals.rays)
-    debug = targets.read()[1]
-    kept = debug[..., 2] != api.RAY_CAPTURED
-    drift = {"budget": float(u.drift_budget), "max_escaped_or_exhausted": float(debug[..., 0][kept].max()),
-             "rays_over_budget_escaped_or_exhausted": int((debug[..., 0][kept] > u.drift_budget).sum()),
-             "rays_over_budget_all": int(totals.over_drift_budget), "captured": int(totals.captured),
-             "exhausted": int(totals.exhausted)}
+    drift = frame_drift(targets, totals)
 
     batched_ms, batched_frames = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, request, cfg), args.warmup,
                                              args.steps)
     ms_per_step = batched_ms / args.steps
     value = steps_per_frame / (ms_per_step * 1e-3)
+    progress(f"headline polar batched {ms_per_step:.3f} ms")
 
+    # The other batched forms on the same frame: the Boyer-Lindquist fast
+    # form (the reference's coordinates, over the drift budget on near-polar
+    # rays as the FP64 oracle is) and the all Kerr-Schild form.
+    side = {}
+    side_steps = max(3, args.steps // 2)
+    for name, ks in (("boyer_lindquist_fast", 0), ("kerr_schild", 1)):
+        c = form_config(ks)
+        targets.reset_counters()
+        t = ctx.march_batch(request, c)
+        d = frame_drift(targets, t)
+        ms, frames = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, request, c), max(1, args.warmup // 2),
+                                 side_steps)
+        side[name] = {"ms_per_step": ms / side_steps, "value": int(t.steps) / (ms / side_steps * 1e-3),
+                      "steps_per_frame": int(t.steps), "drift": d,
+                      "p50_ms": float(np.percentile(frames, 50)), "p99_ms": float(np.percentile(frames, 99))}
+
+    progress("other batched forms")
     plain = None
     if not args.skip_plain:
         plain_targets = api.Targets(ctx, 1920, 1080)
-        plain_targets.reset_counters()
-        ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_REFERENCE)
-        plain_steps = int(plain_targets.counters().steps)
-        plain_ms, plain_frames = time_frames(
-            ctx, api, lambda: ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_REFERENCE),
-            max(1, args.warmup // 2), max(3, args.steps // 2))
-        plain_ms_per = plain_ms / max(3, args.steps // 2)
-        plain = {"ms_per_step": plain_ms_per, "value": plain_steps / (plain_ms_per * 1e-3),
-                 "steps_per_frame": plain_steps, "integrator": "reference", "launch": "one per frame, 8x8 waves",
-                 "p50_ms": float(np.percentile(plain_frames, 50)), "p99_ms": float(np.percentile(plain_frames, 99))}
-        # Decomposition: this library's integrator with the plain launch, so
-        # the gain splits into arithmetic (plain_fast / plain) and scheduling
-        # (batched / plain_fast).
-        plain_targets.reset_counters()
-        ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_FAST)
-        fast_steps = int(plain_targets.counters().steps)
-        fast_ms, _ = time_frames(ctx, api, lambda: ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_FAST),
-                                 max(1, args.warmup // 2), max(3, args.steps // 2))
-        fast_ms_per = fast_ms / max(3, args.steps // 2)
-        plain["fast_integrator"] = {"ms_per_step": fast_ms_per, "value": fast_steps / (fast_ms_per * 1e-3),
-                                    "steps_per_frame": fast_steps}
-        # The Kerr-Schild form (regular on the spin axis) with the plain
-        # launch: its cost, and its drift on the same frame.
-        plain_targets.reset_counters()
-        ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_KERR_SCHILD)
-        ks_totals = plain_targets.counters()
-        ks_debug = plain_targets.read()[1]
-        ks_kept = ks_debug[..., 2] != api.RAY_CAPTURED
-        ks_ms, _ = time_frames(
-            ctx, api, lambda: ctx.march_image(u, plain_targets, vol, None, api.INTEGRATOR_KERR_SCHILD),
-            max(1, args.warmup // 2), max(3, args.steps // 2))
-        ks_ms_per = ks_ms / max(3, args.steps // 2)
-        plain["kerr_schild_integrator"] = {
-            "ms_per_step": ks_ms_per, "value": int(ks_totals.steps) / (ks_ms_per * 1e-3),
-            "steps_per_frame": int(ks_totals.steps),
-            "rays_over_budget_escaped_or_exhausted": int((ks_debug[..., 0][ks_kept] > u.drift_budget).sum()),
-            "max_drift_escaped_or_exhausted": float(ks_debug[..., 0][ks_kept].max())}
+        n_plain = max(3, args.steps // 2)
+
+        def plain_leg(integrator):
+            plain_targets.reset_counters()
+            ctx.march_image(u, plain_targets, vol, None, integrator)
+            c = plain_targets.counters()
+            d = frame_drift(plain_targets, c)
+            ms, frames = time_frames(ctx, api, lambda: ctx.march_image(u, plain_targets, vol, None, integrator),
+                                     max(1, args.warmup // 2), n_plain)
+            per = ms / n_plain
+            return {"ms_per_step": per, "value": int(c.steps) / (per * 1e-3), "steps_per_frame": int(c.steps),
+                    "drift": d, "p50_ms": float(np.percentile(frames, 50)), "p99_ms": float(np.percentile(frames, 99))}
+
+        # The baseline the metric names: the reference's marchImage formulas,
+        # one thread per pixel, 8x8 workgroups, one launch per frame.
+        plain = plain_leg(api.INTEGRATOR_REFERENCE)
+        plain.update({"integrator": "reference", "launch": "one per frame, 8x8 waves"})
+        # This library's forms with the same plain launch, so the gain splits
+        # into arithmetic (plain form / plain reference) and scheduling
+        # (batched / plain of the same form).
+        plain["polar_integrator"] = plain_leg(api.INTEGRATOR_POLAR)
+        plain["fast_integrator"] = plain_leg(api.INTEGRATOR_FAST)
+        plain["kerr_schild_integrator"] = plain_leg(api.INTEGRATOR_KERR_SCHILD)
         plain_targets.close()
-        # The same batched launch running the Kerr-Schild form.
-        ks_cfg = ctx.batch_config(block=cfg.block, waves_per_simd=cfg.waves_per_simd,
-                                  compaction_period=cfg.compaction_period, refill_threshold=cfg.refill_threshold,
-                                  kerr_schild=1)
-        targets.reset_counters()
-        ks_batch = ctx.march_batch(request, ks_cfg)
-        ks_frame = targets.read()[1]
-        ks_frame_kept = ks_frame[..., 2] != api.RAY_CAPTURED
-        ks_batched_ms, ks_frames = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, request, ks_cfg),
-                                               max(1, args.warmup // 2), max(3, args.steps // 2))
-        ks_batched_per = ks_batched_ms / max(3, args.steps // 2)
-        plain["kerr_schild_batched"] = {
-            "ms_per_step": ks_batched_per, "value": int(ks_batch.steps) / (ks_batched_per * 1e-3),
-            "steps_per_frame": int(ks_batch.steps),
-            "speedup_vs_plain": plain_ms_per / ks_batched_per,
-            "rays_over_budget_escaped_or_exhausted": int((ks_frame[..., 0][ks_frame_kept] > u.drift_budget).sum()),
-            "max_drift_escaped_or_exhausted": float(ks_frame[..., 0][ks_frame_kept].max()),
-            "p50_ms": float(np.percentile(ks_frames, 50)), "p99_ms": float(np.percentile(ks_frames, 99))}
+        progress("plain launches")
 
     achieved = steps_per_frame * FLOPS_PER_KERR_STEP / (ms_per_step * 1e-3) / 1e12
     line = {
@@ -242,11 +262,19 @@ def main():
                    "step_cap": 4096, "observer_radius": 50.0, "inclination_deg": 60.0, "fov_deg": 60.0,
                    "particles": 1_000_000, "volume": 128, "volume_half_extent_z": 3.0, "settle_substeps": 600,
                    "rays": rays, "steps_per_frame": steps_per_frame, "compensated": True,
+                   "form": "polar (Boyer-Lindquist fast steps while sin(theta) >= 0.25, Kerr-Schild after)",
                    "batch": {"block": cfg.block, "waves_per_simd": cfg.waves_per_simd,
-                             "compaction_period": cfg.compaction_period, "refill_threshold": cfg.refill_threshold}},
+                             "compaction_period": cfg.compaction_period, "refill_threshold": cfg.refill_threshold,
+                             "kerr_schild": cfg.kerr_schild}},
         "drift": drift,
+        "drift_clause_met": drift["rays_over_budget_escaped_or_exhausted"] == 0,
+        "batched_other_forms": side,
         "plain": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        # Equal integrator: the batched persistent launch against the plain
+        # launch of the same (polar) form; the rest of speedup_vs_plain is
+        # the integrator's arithmetic.
+        "speedup_scheduling": (plain["polar_integrator"]["ms_per_step"] / ms_per_step) if plain else None,
         "latency_ms": {"p50": float(np.percentile(batched_frames, 50)),
                        "p99": float(np.percentile(batched_frames, 99))},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -255,19 +283,23 @@ def main():
                      "peak_nonpacked": FP32_PEAK_NONPACKED_TFLOPS,
                      "flops_per_step": FLOPS_PER_KERR_STEP,
                      "kernel": ("march_tiles_kernel" if cfg.refill_threshold == 0 else "march_batch_kernel")
-                     + f"<1, {cfg.block}, true>"},
+                     + f"<3, {cfg.block}, true>"},
         "device": {"name": info.name.decode(), "arch": info.arch.decode(), "compute_units": info.compute_units},
     }
     if not args.no_mixed:
         line["mixed"] = mixed_leg(ctx, api, vol, args)
+        progress("mixed batch")
     if not args.no_passes:
         line["passes"] = pass_leg(ctx, api, p, vol)
         line["fp64"] = fp64_leg(ctx)
+        progress("disk passes and FP64 kernels")
     if not args.no_mixed:
         line["kerr_640x360_x8"] = kerr_views_leg(ctx, api, vol, cfg)
         line["kerr_presets"] = kerr_presets_leg(ctx, api, vol, cfg)
+        progress("Kerr views and presets")
     if not args.no_schwarzschild:
         line["schwarzschild"] = schwarzschild_legs(ctx, api, vol, cfg, not args.no_cpu_baseline)
+        progress("Schwarzschild legs")
     if not args.no_cpu_baseline:
         every = args.cpu_rows or 1
         line["cpu_baseline"] = cpu_baseline(u, vol, every)
@@ -362,8 +394,9 @@ def kerr_views_leg(ctx, api, vol, cfg, views=8):
     steps = int(totals.steps)
     batched_ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, reqs, cfg), 1, 10)
     out = {"requests": views, "rays": int(totals.rays), "steps": steps, "batched_ms": batched_ms / 10,
-           "batched_steps_per_s": steps / (batched_ms / 10 * 1e-3)}
-    for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
+           "batched_steps_per_s": steps / (batched_ms / 10 * 1e-3), "batched_form": _form_name(cfg)}
+    for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST),
+                             ("plain_polar", api.INTEGRATOR_POLAR)):
         def run_all():
             for r in reqs:
                 ctx.march_image(r["uniforms"], r["targets"], vol, None, integrator)
@@ -371,6 +404,7 @@ def kerr_views_leg(ctx, api, vol, cfg, views=8):
         out[form + "_ms"] = ms / 5
     out["speedup_vs_plain_reference"] = out["plain_reference_ms"] / out["batched_ms"]
     out["speedup_vs_plain_fast"] = out["plain_fast_ms"] / out["batched_ms"]
+    out["speedup_vs_plain_polar"] = out["plain_polar_ms"] / out["batched_ms"]
     for r in reqs:
         r["targets"].close()
     return out
@@ -396,11 +430,14 @@ def kerr_presets_leg(ctx, api, vol, cfg):
         t.reset_counters()
         totals = ctx.march_batch(req, cfg)
         ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, req, cfg), 1, 10)
-        leg = {"rays": int(totals.rays), "steps": int(totals.steps), "batched_ms": ms / 10}
-        for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
+        leg = {"rays": int(totals.rays), "steps": int(totals.steps), "batched_ms": ms / 10,
+               "batched_form": _form_name(cfg)}
+        for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST),
+                                 ("plain_polar", api.INTEGRATOR_POLAR)):
             pms, _ = time_frames(ctx, api, lambda: ctx.march_image(u, t, vol, None, integrator), 1, 5)
             leg[form + "_ms"] = pms / 5
         leg["speedup_vs_plain_reference"] = leg["plain_reference_ms"] / leg["batched_ms"]
+        leg["speedup_vs_plain_polar"] = leg["plain_polar_ms"] / leg["batched_ms"]
         t.close()
         out[name] = leg
     return out
@@ -421,9 +458,18 @@ def bake_leg(ctx, api, vol, tables):
     bake_ms, _ = time_frames(ctx, api, lambda: bake.rows(u, tables), 1, 5)
     walk_ms, _ = time_frames(ctx, api, lambda: bake.walk(u, vol, t), 1, 5)
     headers, _ = bake.read()
+    # The walk's algorithmic HBM bytes per ray: the 16-byte header, 8 bytes
+    # per stored sample it reads (at most the ray's count; it stops at the
+    # transmittance floor) and 12 bytes written (colour and depth). The
+    # volume's texels are L2-resident reads, not counted.
+    walk_bytes = int(w * h * (16 + 12) + 8 * int(headers[:, 0].sum()))
     out = {"bake_ms": bake_ms / 5, "walk_ms": walk_ms / 5, "rays": w * h, "overflow_rays": bake.overflow(),
            "mean_samples_per_ray": float(headers[:, 0].mean()),
-           "bake_bytes": w * h * (8 * 96 + 16)}
+           "bake_bytes": w * h * (8 * 96 + 16),
+           "walk_bytes_upper": walk_bytes, "walk_GB_per_s": walk_bytes / (walk_ms / 5 * 1e-3) / 1e9,
+           "walk_hbm_frac": walk_bytes / (walk_ms / 5 * 1e-3) / 8e12,
+           # The bake writes its samples and headers: 8 bytes per stored sample, 16 per ray.
+           "bake_written_bytes": int(16 * w * h + 8 * int(headers[:, 0].sum()))}
     bake.close()
     t.close()
     return out
@@ -468,7 +514,10 @@ def mixed_leg(ctx, api, vol, args):
     requests marched one plain launch each, both integrator forms."""
     from gargantua_amd import autotune, workloads
     reqs = workloads.mixed(ctx, vol)
-    cfg = autotune.best(ctx, "mixed")
+    tuned = autotune.best(ctx, "mixed")
+    cfg = ctx.batch_config(block=tuned.block, waves_per_simd=tuned.waves_per_simd,
+                           compaction_period=tuned.compaction_period, refill_threshold=tuned.refill_threshold,
+                           kerr_schild=0)
     for r in reqs:
         r["targets"].reset_counters()
     totals = ctx.march_batch(reqs, cfg)
@@ -479,7 +528,24 @@ def mixed_leg(ctx, api, vol, args):
            "batch": {"block": cfg.block, "waves_per_simd": cfg.waves_per_simd,
                      "compaction_period": cfg.compaction_period, "refill_threshold": cfg.refill_threshold},
            "batched_ms": batched_ms / 3, "batched_steps_per_s": steps / (batched_ms / 3 * 1e-3)}
-    for name, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
+    # The same batch in the polar form, with each request's drift budget.
+    polar = ctx.batch_config(block=cfg.block, waves_per_simd=cfg.waves_per_simd,
+                             compaction_period=cfg.compaction_period, refill_threshold=cfg.refill_threshold,
+                             kerr_schild=2)
+    for r in reqs:
+        r["targets"].reset_counters()
+    ptotals = ctx.march_batch(reqs, polar)
+    over = 0
+    for r in reqs:
+        debug = r["targets"].read()[1]
+        kept = debug[..., 2] != api.RAY_CAPTURED
+        over += int((debug[..., 0][kept] > r["uniforms"].drift_budget).sum())
+    polar_ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, reqs, polar), 1, 3)
+    out["polar"] = {"batched_ms": polar_ms / 3, "steps": int(ptotals.steps),
+                    "batched_steps_per_s": int(ptotals.steps) / (polar_ms / 3 * 1e-3),
+                    "rays_over_budget_escaped_or_exhausted": over}
+    for name, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST),
+                             ("plain_polar", api.INTEGRATOR_POLAR)):
         def run_all():
             for r in reqs:
                 ctx.march_image(r["uniforms"], r["targets"], vol, None, integrator)
@@ -488,6 +554,8 @@ def mixed_leg(ctx, api, vol, args):
         out[name + "_steps_per_s"] = steps / (ms / 2 * 1e-3)
     out["speedup_vs_plain_reference"] = out["plain_reference_ms"] / out["batched_ms"]
     out["speedup_vs_plain_fast"] = out["plain_fast_ms"] / out["batched_ms"]
+    out["polar"]["speedup_vs_plain_reference"] = out["plain_reference_ms"] / out["polar"]["batched_ms"]
+    out["polar"]["speedup_vs_plain_polar"] = out["plain_polar_ms"] / out["polar"]["batched_ms"]
     workloads.close(reqs)
     return out
 
@@ -530,7 +598,9 @@ def pass_leg(ctx, api, particles, vol):
     # The ceiling for the bin: the same count of no-return atomics (8 corners
     # x 6 sums per particle) to random words of the sums' footprint.
     ceiling = ctx.atomic_rate(6 * voxels, 48, n)
-    return {"sim": {"ms": sim_ms, "particles_per_s": n / (sim_ms * 1e-3), "bytes": sim_bytes,
+    max_preset = max_preset_passes(ctx, api, timed)
+    return {"max_preset_4M_256": max_preset,
+            "sim": {"ms": sim_ms, "particles_per_s": n / (sim_ms * 1e-3), "bytes": sim_bytes,
                     "GB_per_s": sim_bytes / (sim_ms * 1e-3) / 1e9, "hbm_frac": sim_bytes / (sim_ms * 1e-3) / 8e12,
                     "fused_50": {"ms_per_dispatch": fused_ms, "particle_dispatches_per_s": n / (fused_ms * 1e-3)}},
             # Exact bin: 8 corners x 6 integer sums per particle. Sorted and
@@ -545,6 +615,41 @@ def pass_leg(ctx, api, particles, vol):
             "resolve": {"ms": resolve_ms, "voxels_per_s": voxels / (resolve_ms * 1e-3), "bytes": resolve_bytes,
                         "GB_per_s": resolve_bytes / (resolve_ms * 1e-3) / 1e9,
                         "hbm_frac": resolve_bytes / (resolve_ms * 1e-3) / 8e12}}
+
+
+def max_preset_passes(ctx, api, timed):
+    """The serving disk state (4M particles, 256^3): one sim dispatch (64
+    bytes per particle), the stochastic splat that serving runs every frame
+    (one corner of six no-return atomics per particle, 28 bytes read) and
+    the resolve with its fused clear (64 bytes per voxel)."""
+    n, size = 4_000_000, 256
+    voxels = size ** 3
+    p = api.Particles(ctx, n, seed=1)
+    p.seed(api.GENERATOR_DISK)
+    p.step(20)
+    v = api.Volume(ctx, size)
+    v.clear()
+    sim_ms = timed(lambda: p.step(1), 20)
+    frame = [0]
+
+    def stochastic():
+        frame[0] += 1
+        v.splat(p, exact=False, frame=frame[0])
+
+    splat_ms = timed(stochastic, 20)
+    resolve_ms = timed(lambda: (stochastic(), v.resolve(clear=True)), 10) - splat_ms
+    ceiling = ctx.atomic_rate(6 * voxels, 6, n)
+    out = {"sim": {"ms": sim_ms, "bytes": 64 * n, "GB_per_s": 64 * n / (sim_ms * 1e-3) / 1e9,
+                   "hbm_frac": 64 * n / (sim_ms * 1e-3) / 8e12},
+           "splat_stochastic": {"ms": splat_ms, "deposits_per_s": n / (splat_ms * 1e-3),
+                                "atomics_per_s": 6 * n / (splat_ms * 1e-3), "atomic_ceiling_per_s": ceiling,
+                                "vs_atomic_ceiling": 6 * n / (splat_ms * 1e-3) / ceiling,
+                                "bytes_read": 28 * n, "GB_per_s_read": 28 * n / (splat_ms * 1e-3) / 1e9},
+           "resolve": {"ms": resolve_ms, "bytes": 64 * voxels, "GB_per_s": 64 * voxels / (resolve_ms * 1e-3) / 1e9,
+                       "hbm_frac": 64 * voxels / (resolve_ms * 1e-3) / 8e12}}
+    v.close()
+    p.close()
+    return out
 
 
 def fp64_leg(ctx):
@@ -562,6 +667,10 @@ def fp64_leg(ctx):
     return {"shadow_boundary": {"azimuths": 360, "ms": shadow_ms, "width_rad": float(x.max() - x.min()),
                                 "centre_offset_rad": float(0.5 * (x.max() + x.min()))},
             "far_field_table": {"entries": 4096, "ms": table_ms}}
+
+
+def _form_name(cfg):
+    return {0: "boyer_lindquist_fast", 1: "kerr_schild", 2: "polar"}.get(int(cfg.kerr_schild), "?")
 
 
 def _batch_nosync(ctx, api, request, cfg):

@@ -1,7 +1,9 @@
 """The batch autotuner: sweeps block size, resident waves per SIMD,
 compaction period and refill threshold of the persistent march on a
-workload, and persists the fastest configuration per (arch, workload) in
-gargantua_amd/autotune.json, which callers load with `best()`.
+workload, and persists the fastest configuration per (arch, workload) as
+<table>/<arch>/<workload>.json = {"config": {...}, "ms": ..., "date": ...},
+which callers load with `best()`. The table is GARG_AUTOTUNE_DIR, else
+gargantua_amd/autotune; `gargantua bench --sweep-batch` writes the same files.
 
 python -m gargantua_amd.autotune --workload headline|mixed
 """
@@ -15,7 +17,12 @@ import time
 
 from . import api, workloads
 
-PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune.json")
+def table_dir() -> str:
+    return os.environ.get("GARG_AUTOTUNE_DIR") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune")
+
+
+def entry_path(arch: str, workload: str) -> str:
+    return os.path.join(table_dir(), arch, f"{workload}.json")
 
 # refill_threshold 0 selects whole-tile claims, where the compaction period
 # plays no part; the grid visits it once per block size and wave count.
@@ -51,7 +58,11 @@ def _enqueue(ctx, requests, cfg):
         raise api.GargError(st, api.load().garg_last_error().decode())
 
 
-def sweep(ctx: api.Context, requests: list, grid: dict | None = None, repeats: int = 2, log=print) -> list:
+FORMS = {"fast": 0, "kerr-schild": 1, "polar": 2}  # garg_batch_config.kerr_schild
+
+
+def sweep(ctx: api.Context, requests: list, grid: dict | None = None, repeats: int = 2, log=print,
+          kerr_schild: int = 2) -> list:
     grid = grid or GRID
     results = []
     seen = set()
@@ -60,36 +71,37 @@ def sweep(ctx: api.Context, requests: list, grid: dict | None = None, repeats: i
             if (block, waves) in seen:
                 continue
             seen.add((block, waves))
-        cfg = ctx.batch_config(block=block, waves_per_simd=waves, compaction_period=period, refill_threshold=thr)
+        cfg = ctx.batch_config(block=block, waves_per_simd=waves, compaction_period=period, refill_threshold=thr,
+                               kerr_schild=kerr_schild)
         ms = time_config(ctx, requests, cfg, repeats)
         results.append(({"block": block, "waves_per_simd": waves, "compaction_period": period,
-                          "refill_threshold": thr}, ms))
+                          "refill_threshold": thr, "kerr_schild": kerr_schild}, ms))
         log(f"block {block:3d} waves {waves} period {period:3d} threshold {thr:2d}: {ms:9.3f} ms")
     results.sort(key=lambda r: r[1])
     return results
 
 
+def load_entry(arch: str, workload: str) -> dict | None:
+    try:
+        with open(entry_path(arch, workload)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def best(ctx: api.Context, workload: str) -> api.BatchConfig:
     """The persisted winner for this device and workload, else the default."""
-    arch = ctx.info().arch.decode()
-    try:
-        table = json.load(open(PATH))
-        entry = table.get(arch, {}).get(workload)
-        if entry:
-            return ctx.batch_config(**entry["config"])
-    except (OSError, ValueError):
-        pass
+    entry = load_entry(ctx.info().arch.decode(), workload)
+    if entry and isinstance(entry.get("config"), dict):
+        return ctx.batch_config(**entry["config"])
     return ctx.batch_config()
 
 
 def save(arch: str, workload: str, config: dict, ms: float):
-    try:
-        table = json.load(open(PATH))
-    except (OSError, ValueError):
-        table = {}
-    table.setdefault(arch, {})[workload] = {"config": config, "ms": ms, "date": time.strftime("%Y-%m-%d")}
-    with open(PATH, "w") as f:
-        json.dump(table, f, indent=1, sort_keys=True)
+    path = entry_path(arch, workload)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump({"config": config, "ms": ms, "date": time.strftime("%Y-%m-%d")}, f, indent=1, sort_keys=True)
 
 
 def main():
@@ -97,6 +109,7 @@ def main():
     ap.add_argument("--workload", choices=["headline", "mixed"], default="headline")
     ap.add_argument("--repeats", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="a reduced grid")
+    ap.add_argument("--form", choices=sorted(FORMS), default="polar", help="integrator form of spinning requests")
     args = ap.parse_args()
     ctx = api.Context()
     p, v = workloads.disk(ctx)
@@ -105,7 +118,7 @@ def main():
     if args.quick:
         grid = {"block": [64, 256], "waves_per_simd": [4, 5], "compaction_period": [16, 64],
                 "refill_threshold": [0, 32, 63]}
-    results = sweep(ctx, reqs, grid, args.repeats)
+    results = sweep(ctx, reqs, grid, args.repeats, kerr_schild=FORMS[args.form])
     cfg, ms = results[0]
     save(ctx.info().arch.decode(), args.workload, cfg, ms)
     print(json.dumps({"workload": args.workload, "best": cfg, "ms": ms, "worst_ms": results[-1][1]}))

@@ -21,7 +21,7 @@ GARG_DEV float2 unpack_half2(uint32_t v) {
     return make_float2((float)a, (float)b);
 }
 
-// bakeRays (bake.metal:216-291): the still-policy Schwarzschild march that
+// bakeRays (bake.metal:17-92): the still-policy Schwarzschild march that
 // stores half4(position, path since the last sample) at least bake_spacing
 // apart while inside the slab, up to `capacity` samples.
 __global__ __launch_bounds__(64) void bake_kernel(BakeArgs a) {
@@ -96,28 +96,45 @@ __global__ __launch_bounds__(64) void bake_kernel(BakeArgs a) {
     if (overflowed) atomicAdd(a.overflow, 1u);
 }
 
-// walkRays (bake.metal:293-343): composite the stored samples against the
+// walkRays (bake.metal:94-144): composite the stored samples against the
 // current volume with g^3 (no beaming floor, as the reference walk does),
-// then the starfield along the stored direction.
+// then the starfield along the stored direction. A wave walks 64
+// consecutive pixels of a row, so sample k of its 64 rays is 512 contiguous
+// bytes of the k-th slab. The walk is a chain of dependent loads (a stored
+// point, then its eight texels), so it is software-pipelined: while sample
+// k is composited, the texels of sample k + 1 and the point of sample k + 2
+// are already in flight. The arithmetic of each sample is sample_volume's,
+// in the same order.
 __global__ __launch_bounds__(64) void walk_kernel(WalkArgs a) {
     const garg_march_uniforms& u = a.uniforms;
-    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
-    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    uint32_t px = blockIdx.x * 64u + threadIdx.x;
+    uint32_t py = blockIdx.y;
     if (px >= u.resolution[0] || py >= u.resolution[1]) return;
     const uint32_t rays = u.resolution[0] * u.resolution[1];
     const uint32_t index = py * u.resolution[0] + px;
-    uint4 header = a.headers[index];
+    const uint4 header = a.headers[index];
+    const uint32_t count = min(header.x, a.capacity);
     f3 color{0, 0, 0};
     float transmittance = 1.0f;
     f3 prev = load3(u.camera_position);
     const bool gas = a.scene.volume.voxels != nullptr;
-    for (uint32_t k = 0; k < a.capacity; ++k) {
-        if (k >= header.x || transmittance <= GARG_F(TRANSMITTANCE_FLOOR)) break;
-        h4 stored = a.samples[(size_t)k * rays + index];
-        f3 pos{(float)stored.x, (float)stored.y, (float)stored.z};
-        float path = (float)stored.w;
-        if (gas) {
-            Sample s = sample_volume(a.scene.volume, pos);
+    if (gas && count > 0u) {
+        const h4* slab = a.samples + index;
+        h4 point = slab[0];
+        TexelCorners texels = fetch_corners(a.scene.volume, f3{(float)point.x, (float)point.y, (float)point.z});
+        h4 ahead = count > 1u ? slab[(size_t)rays] : point;
+        for (uint32_t k = 0; k < count; ++k) {
+            if (transmittance <= GARG_F(TRANSMITTANCE_FLOOR)) break;
+            const h4 cur = point;
+            const TexelCorners here = texels;
+            if (k + 1u < count) {
+                point = ahead;
+                texels = fetch_corners(a.scene.volume, f3{(float)point.x, (float)point.y, (float)point.z});
+                if (k + 2u < count) ahead = slab[(size_t)(k + 2u) * rays];
+            }
+            const f3 pos{(float)cur.x, (float)cur.y, (float)cur.z};
+            const float path = (float)cur.w;
+            const Sample s = combine_corners(here);
             if (s.density > 0.0f) {
                 float g3 = 1.0f;
                 if (u.redshift != 0u) {
@@ -128,8 +145,8 @@ __global__ __launch_bounds__(64) void walk_kernel(WalkArgs a) {
                 color = color + s.emission * (transmittance * g3 * path);
                 transmittance *= 1.0f - alpha;
             }
+            prev = pos;
         }
-        prev = pos;
     }
     float2 xy = unpack_half2(header.z), zd = unpack_half2(header.w);
     if (header.y == 0u && u.star_brightness > 0.0f) {
@@ -147,7 +164,7 @@ hipError_t launch_bake(const BakeArgs& a, hipStream_t stream) {
 }
 
 hipError_t launch_walk(const WalkArgs& a, hipStream_t stream) {
-    dim3 grid((a.uniforms.resolution[0] + 7u) / 8u, (a.uniforms.resolution[1] + 7u) / 8u);
+    dim3 grid((a.uniforms.resolution[0] + 63u) / 64u, a.uniforms.resolution[1]);
     walk_kernel<<<grid, 64, 0, stream>>>(a);
     return hipGetLastError();
 }

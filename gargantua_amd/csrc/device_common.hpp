@@ -109,6 +109,59 @@ GARG_DEV Sample sample_volume(const VolumeView& v, f3 p) {
     return s;
 }
 
+// sample_volume in two halves, so a caller can keep one sample's texel
+// loads in flight while it composites the previous one: fetch_corners
+// issues the eight corner loads (from in-range indices; a corner outside
+// the volume or of zero weight is dropped by combine_corners, as
+// sample_volume skips it), combine_corners forms the same sums in the same
+// order as sample_volume.
+struct TexelCorners {
+    Voxel v[8];
+    float fx, fy, fz;
+    uint32_t keep;  // bit c: corner c is inside the volume
+};
+
+GARG_DEV TexelCorners fetch_corners(const VolumeView& v, f3 p) {
+    TexelCorners c;
+    float tx = (p.x + v.half_xy) * v.scale_xy - 0.5f;
+    float ty = (p.y + v.half_xy) * v.scale_xy - 0.5f;
+    float tz = (p.z + v.half_z) * v.scale_z - 0.5f;
+    float bx = floorf(tx), by = floorf(ty), bz = floorf(tz);
+    c.fx = tx - bx;
+    c.fy = ty - by;
+    c.fz = tz - bz;
+    int x0 = (int)bx, y0 = (int)by, z0 = (int)bz;
+    const __attribute__((address_space(1))) Voxel* g = (const __attribute__((address_space(1))) Voxel*)v.voxels;
+    c.keep = 0u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int x = x0 + (k & 1), y = y0 + ((k >> 1) & 1), z = z0 + ((k >> 2) & 1);
+        bool inside = (unsigned)x < (unsigned)v.size && (unsigned)y < (unsigned)v.size && (unsigned)z < (unsigned)v.size;
+        c.keep |= inside ? (1u << k) : 0u;
+        uint32_t i = inside ? ((uint32_t)z * (uint32_t)v.size + (uint32_t)y) * (uint32_t)v.size + (uint32_t)x : 0u;
+        c.v[k].emission = g[i].emission;
+        c.v[k].velocity = g[i].velocity;
+    }
+    return c;
+}
+
+GARG_DEV Sample combine_corners(const TexelCorners& c) {
+    Sample s{f3{0, 0, 0}, 0.0f, f3{0, 0, 0}};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float w = ((k & 1) ? c.fx : 1.0f - c.fx) * (((k >> 1) & 1) ? c.fy : 1.0f - c.fy) *
+                  (((k >> 2) & 1) ? c.fz : 1.0f - c.fz);
+        if (((c.keep >> k) & 1u) && w != 0.0f) {
+            const h4 em = c.v[k].emission;
+            const h4 ve = c.v[k].velocity;
+            s.emission = s.emission + f3{(float)em.x, (float)em.y, (float)em.z} * w;
+            s.density += (float)em.w * w;
+            s.velocity = s.velocity + f3{(float)ve.x, (float)ve.y, (float)ve.z} * w;
+        }
+    }
+    return s;
+}
+
 // Linear lookup in an n entry table at a normalized coordinate with texel
 // centres at (i + 0.5) / n, clamped to the edge entries.
 GARG_DEV float table_linear(const float* table, int n, float coordinate) {

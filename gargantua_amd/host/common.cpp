@@ -1,6 +1,7 @@
 // Host pieces shared by the commands: exits, the option grammar, presets and
 // budget, the orbit camera and uniforms, the PNG writer and the scene setup.
 #include <cmath>
+#include <dlfcn.h>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -34,7 +35,7 @@ void check(int status, const char* what) {
 const char* kUsage =
     "usage: gargantua [--preset small|default|max] [--particles N] [--volume 96|128|192|256] [--seed S] "
     "[--strategy march|bake] [--fps-cap 30|60] [--no-upscale] [--spin A] [--kerr-schild] [--camera az,el,dist] "
-    "[--still out.png --width W --height H [--sequence N]] | bench [--sweep] | validate | serve [--rate R] "
+    "[--still out.png --width W --height H [--sequence N]] | bench [--sweep | --sweep-batch] | validate | serve [--rate R] "
     "[--seconds T] [--deadline D] [--frames F] [--tick-rays N] [--policy srpt|fifo] [--bake-share F] [--max-wait S] "
     "[--verify N] [--report PATH] | version | constants";
 
@@ -127,6 +128,8 @@ std::string parse_options(const std::vector<std::string>& args, Options& o) {
             o.soak_seconds = s;
         } else if (a == "--sweep") {
             o.sweep = true;
+        } else if (a == "--sweep-batch") {
+            o.sweep_batch = true;
         } else if (a == "--sequence") {
             int n;
             if (!(err = integer(a, n)).empty()) return err;
@@ -294,6 +297,32 @@ garg_geodesic_settings policy_settings(Policy p) {
     return s;
 }
 
+// Probes.cameraGrid (Validation/Probes.swift:52-66): the camera's rays
+// through the centres of a columns x rows grid over the field of view.
+std::vector<garg_ray_probe> camera_grid(const OrbitCamera& cam, int columns, int rows, double aspect) {
+    float pos[3], right[3], up[3], fwd[3];
+    cam.basis(pos, right, up, fwd);
+    double tan_half = std::tan(cam.fov_y * 0.5f);
+    std::vector<garg_ray_probe> out;
+    for (int row = 0; row < rows; ++row)
+        for (int col = 0; col < columns; ++col) {
+            double x = (2.0 * (col + 0.5) / columns - 1.0) * tan_half * aspect;
+            double y = (1.0 - 2.0 * (row + 0.5) / rows) * tan_half;
+            double d[3], len = 0;
+            for (int k = 0; k < 3; ++k) {
+                d[k] = fwd[k] + x * right[k] + y * up[k];
+                len += d[k] * d[k];
+            }
+            garg_ray_probe p;
+            for (int k = 0; k < 3; ++k) {
+                p.origin[k] = pos[k];
+                p.direction[k] = (float)(d[k] / std::sqrt(len));
+            }
+            out.push_back(p);
+        }
+    return out;
+}
+
 // MarchPass.uniforms (MarchPass.swift:67-98) without the table fields.
 garg_march_uniforms make_uniforms(const OrbitCamera& camera, int width, int height, Policy policy, bool redshift,
                                   uint32_t star_seed, bool stars, double spin, bool compensated) {
@@ -449,6 +478,19 @@ Scene make_scene(const Configuration& c, int settle_substeps, bool exact_splat) 
     check(garg_tables_create(s.ctx, nullptr, GARG_F(VOLUME_HALF_EXTENT), &s.tables), "building sweep tables");
     check(garg_stream_sync(s.ctx, nullptr), "settling the scene");
     return s;
+}
+
+// Where the batch autotuner's table lives: GARG_AUTOTUNE_DIR, else the
+// autotune directory beside libgargantua_hip.so.
+std::string autotune_dir() {
+    if (const char* env = getenv("GARG_AUTOTUNE_DIR"); env && *env) return env;
+    Dl_info info;
+    if (dladdr((void*)&garg_version, &info) && info.dli_fname) {
+        std::string lib = info.dli_fname;
+        size_t slash = lib.find_last_of('/');
+        return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/autotune";
+    }
+    return "autotune";
 }
 
 }  // namespace host

@@ -38,7 +38,8 @@ struct Options {
     int width = 3840;
     int height = 2160;
     std::optional<double> soak_seconds;
-    bool sweep = false;
+    bool sweep = false;        // bench: the reference's step-policy, splat and launch-shape sweep
+    bool sweep_batch = false;  // bench: the batch autotuner over the persistent march's launch
     std::optional<int> sequence;
     std::optional<double> cam_azimuth, cam_elevation, cam_distance;
     double spin = 0.0;
@@ -99,6 +100,11 @@ struct OrbitCamera {
     void basis(float position[3], float right[3], float up[3], float forward[3]) const;
 };
 OrbitCamera camera_from(const Options& o);
+
+// Probes.cameraGrid (Validation/Probes.swift:52-66).
+std::vector<garg_ray_probe> camera_grid(const OrbitCamera& cam, int columns, int rows, double aspect);
+// The batch autotuner's table directory (GARG_AUTOTUNE_DIR, else beside the library).
+std::string autotune_dir();
 
 enum class Policy { interactive, still };
 garg_geodesic_settings policy_settings(Policy p);

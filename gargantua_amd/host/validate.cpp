@@ -144,30 +144,6 @@ garg_ray_probe aimed(double radius, double b) {
     return p;
 }
 
-std::vector<garg_ray_probe> camera_grid(const OrbitCamera& cam, int columns, int rows, double aspect) {
-    float pos[3], right[3], up[3], fwd[3];
-    cam.basis(pos, right, up, fwd);
-    double tan_half = std::tan(cam.fov_y * 0.5f);
-    std::vector<garg_ray_probe> out;
-    for (int row = 0; row < rows; ++row)
-        for (int col = 0; col < columns; ++col) {
-            double x = (2.0 * (col + 0.5) / columns - 1.0) * tan_half * aspect;
-            double y = (1.0 - 2.0 * (row + 0.5) / rows) * tan_half;
-            double d[3], len = 0;
-            for (int k = 0; k < 3; ++k) {
-                d[k] = fwd[k] + x * right[k] + y * up[k];
-                len += d[k] * d[k];
-            }
-            garg_ray_probe p;
-            for (int k = 0; k < 3; ++k) {
-                p.origin[k] = pos[k];
-                p.direction[k] = (float)(d[k] / std::sqrt(len));
-            }
-            out.push_back(p);
-        }
-    return out;
-}
-
 std::vector<garg_ray_probe_result> probe(garg_ctx* ctx, const std::vector<garg_ray_probe>& in,
                                          const garg_geodesic_settings& s, double spin, garg_tables* tables) {
     OrbitCamera cam;

@@ -14,8 +14,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "gargantua_amd", "bin", "gargantua")
 
 
-def run(*args, timeout=600):
-    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+def run(*args, timeout=600, cwd=ROOT, env=None):
+    full = dict(os.environ, **(env or {}))
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=timeout, cwd=cwd, env=full)
 
 
 @pytest.mark.parametrize("args,code,sentence", [
@@ -162,3 +163,65 @@ def test_serve_sheds_what_it_cannot_finish_in_time(tmp_path):
     rep = json.loads(report.read_text())
     assert rep["shed"] > 0 and rep["unfinished"] == 0
     assert rep["completed"] + rep["shed"] + rep["refused"] == rep["requests"]
+
+
+# The report of Bench.swift:236-264 (bench/results/2026-08-23-204508.json):
+# its keys, the three statistics of every pass, and the two frame figures.
+REPORT_KEYS = {"bake", "date", "device", "frames", "iterations", "march", "os", "output", "particles", "passes",
+               "preset", "verdict", "version", "volume"}
+
+
+@pytest.mark.gpu
+def test_bench_writes_the_reference_report(tmp_path):
+    r = run("bench", "--preset", "small", cwd=str(tmp_path))
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    reports = list((tmp_path / "bench" / "results").glob("*.json"))
+    assert len(reports) == 1
+    rep = json.loads(reports[0].read_text())
+    assert REPORT_KEYS <= set(rep), REPORT_KEYS - set(rep)
+    for name in ("sim", "splat", "march", "bake", "walk", "present"):
+        assert set(rep["passes"][name]) == {"mean_ms", "p50_ms", "max_ms"}
+        assert 0 < rep["passes"][name]["p50_ms"] <= rep["passes"][name]["max_ms"]
+    assert set(rep["frames"]) == {"march_ms", "bake_ms"} and set(rep["bake"]) == {"bytes", "overflow_rays"}
+    assert rep["iterations"] == 30 and rep["preset"] == "small" and rep["march"] == [640, 360]
+    assert rep["verdict"].startswith("march frame") and "bake frame" in rep["verdict"]
+
+
+@pytest.mark.gpu
+def test_bench_sweep_is_the_reference_sweep():
+    """Bench.swift:49-124: fifteen step policies (march time, probe drift,
+    exhausted probes, image difference from the spec policy), the splat's
+    phases, then the launch shapes."""
+    r = run("bench", "--preset", "small", "--sweep")
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [l for l in r.stdout.splitlines() if l.startswith(("fixed", "prop"))]
+    assert len(rows) == 15
+    spec = [l for l in rows if "(spec)" in l][0].split()
+    assert float(spec[-1]) == 0.0  # the spec policy against itself
+    assert "bin " in r.stdout and "resolve " in r.stdout and r.stdout.count("waves/SIMD  march") == 9
+
+
+@pytest.mark.gpu
+def test_bench_sweep_batch_writes_the_table_the_autotuner_reads(tmp_path, ctx):
+    from gargantua_amd import autotune
+    r = run("bench", "--preset", "small", "--sweep-batch", env={"GARG_AUTOTUNE_DIR": str(tmp_path)})
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    arch = ctx.info().arch.decode()
+    entry = json.loads((tmp_path / arch / "headline.json").read_text())
+    assert entry["ms"] > 0 and entry["config"]["kerr_schild"] == 2
+    old = os.environ.get("GARG_AUTOTUNE_DIR")
+    os.environ["GARG_AUTOTUNE_DIR"] = str(tmp_path)
+    try:
+        cfg = autotune.best(ctx, "headline")
+    finally:
+        if old is None:
+            del os.environ["GARG_AUTOTUNE_DIR"]
+        else:
+            os.environ["GARG_AUTOTUNE_DIR"] = old
+    for k, v in entry["config"].items():
+        assert getattr(cfg, k) == v, k
+    # Threshold 0 (whole tiles) and five waves per SIMD are in the grid.
+    assert " 0  " in r.stdout and any(l.split()[:2] == ["256", "5"] for l in r.stdout.splitlines() if l[:3].strip().isdigit())
