@@ -199,7 +199,7 @@ def settings(kind: str = "still", **overrides) -> GeodesicSettings:
 
 def orbit_camera(azimuth: float = 0.6, elevation: float = 0.32, distance: float = 52.0,
                  fov_y_deg: float = 50.0):
-    """OrbitCamera basis (Camera.swift:256-285) in float32 like the reference."""
+    """OrbitCamera basis (Camera.swift:26-33) in float32 like the reference."""
     az, el, dist = np.float32(azimuth), np.float32(elevation), np.float32(distance)
     c = np.cos(el, dtype=np.float32)
     pos = (dist * np.array([c * np.cos(az), c * np.sin(az), np.sin(el)], dtype=np.float32)).astype(np.float32)

@@ -709,7 +709,7 @@ int garg_tables_create(garg_ctx* ctx, garg_stream* stream, float sphere_radius, 
     t->ctx = ctx;
     t->sphere_radius = sphere_radius;
     // Rays whose periapsis lies outside r = R - 1 never enter, so no ray is
-    // placed on the sphere grazing it (DeflectionTables.swift:193-202).
+    // placed on the sphere grazing it (DeflectionTables.swift:24-31).
     t->entry_radius = (double)sphere_radius - 1.0;
     t->camera_radius = 0.0;
     int rc = ctx_alloc(ctx, 3u * SWEEP_TABLE_SIZE * 4u, (void**)&t->data, "the sweep tables");

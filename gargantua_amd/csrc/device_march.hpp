@@ -368,7 +368,7 @@ GARG_DEV float kerr_sigma(float a, float r, float theta) {
     return r * r + a * a * c * c;
 }
 
-// Boyer-Lindquist radius of a Cartesian point (Kerr.h:339-344).
+// Boyer-Lindquist radius of a Cartesian point (Kerr.h:101-106).
 GARG_DEV float kerr_radius_of(float a, f3 p) {
     float a2 = a * a;
     float rho2 = dot(p, p);
@@ -377,7 +377,7 @@ GARG_DEV float kerr_radius_of(float a, f3 p) {
 }
 
 // Launch from a static observer tetrad along a locally measured direction
-// decomposed against the spherical frame (Kerr.h:348-386).
+// decomposed against the spherical frame (Kerr.h:110-148).
 GARG_DEV KerrRayState kerr_launch(float a, f3 origin, f3 direction) {
     float r = kerr_radius_of(a, origin);
     float rho = length(origin);
@@ -422,7 +422,7 @@ GARG_DEV f3 kerr_position_sc(float a, float r, float st, float ct, float phi) {
 }
 
 // Direction of travel against the spherical frame from the coordinate
-// velocities (Kerr.h:396-411).
+// velocities (Kerr.h:158-173).
 GARG_DEV f3 kerr_direction_from(float r, float st, float ct, float phi, float rdot, float thdot, float phdot) {
     float sp, cp;
     sincosf(phi, &sp, &cp);
@@ -526,7 +526,7 @@ GARG_DEV f3 direction(float a, const KerrRayState& s) {
     return kerr_direction_from(s.r, sinf(s.theta), cosf(s.theta), s.phi, rdot, thdot, phdot);
 }
 
-// Kerr.h:415-424.
+// Kerr.h:177-186.
 GARG_DEV float step_length(float a, const KerrRayState& s, const garg_geodesic_settings& g) {
     float near_h = 0.2f * (s.r - kerr_outer_horizon(a));
     float pole = fminf(fmaxf(sinf(s.theta) / GARG_F(KERR_POLE_SINE), 0.05f), 1.0f);

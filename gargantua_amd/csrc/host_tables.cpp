@@ -1,7 +1,7 @@
 // Host-built FP64 tables the kernels read, made once per context the way the
 // reference builds them at startup (Sim/ParticleSystem.swift:17-48): the
-// blackbody colour table (Oracle/Blackbody.swift:184-256) and the inverse
-// CDF of the steady surface density profile (Oracle/DiskModel.swift:304-351).
+// blackbody colour table (Oracle/Blackbody.swift:13-82) and the inverse
+// CDF of the steady surface density profile (Oracle/DiskModel.swift:31-78).
 #include "host_tables.hpp"
 
 #include <algorithm>
@@ -103,7 +103,7 @@ void build_inverse_cdf(std::vector<float>& table) {
 
 double temperature_scale() {
     // T_peak over the unnormalized profile at its peak, r = 49/36 r_isco
-    // (Sim/ParticleSystem.swift:279-290).
+    // (Sim/ParticleSystem.swift:59-70).
     double x = 36.0 / 49.0;
     double shape = std::pow(x, 0.75) * std::pow(std::max(1.0 - std::sqrt(x), ZERO_TORQUE_FLOOR), 0.25);
     return T_PEAK_DEFAULT / shape;

@@ -17,7 +17,7 @@ GARG_DEV void add_i32(int32_t* p, int32_t v) {
     if (v) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Fixed point round to nearest: uint(x + 0.5) and rint (splat.metal:104-116).
+// Fixed point round to nearest: uint(x + 0.5) and rint (splat.metal:25-37).
 GARG_DEV void deposit(const SplatArgs& a, uint32_t index, float weight, float energy, float relative, float vx,
                       float vy, float vz) {
     float e = energy * weight;
@@ -38,7 +38,7 @@ GARG_DEV void deposit(const SplatArgs& a, uint32_t index, float weight, float en
     add_u32(&a.mass[index], (uint32_t)m);
 }
 
-// Cloud in cell in the space of voxel centres (splat.metal:123-167).
+// Cloud in cell in the space of voxel centres (splat.metal:44-88).
 __global__ __launch_bounds__(256) void splat_kernel(SplatArgs a) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= a.p.count) return;
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs a) {
 }
 
 // Sums to emission per unit volume (tint from the mean temperature) with the
-// density in alpha, and mean velocity (splat.metal:172-199). With `clear`
+// density in alpha, and mean velocity (splat.metal:93-120). With `clear`
 // the sums are zeroed in the same pass, which replaces the separate fill.
 __global__ __launch_bounds__(256) void resolve_kernel(ResolveArgs a) {
     uint32_t voxels = a.size * a.size * a.size;

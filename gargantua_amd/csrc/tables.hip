@@ -67,7 +67,7 @@ GARG_DEV double block_sweep(double r, double b, uint32_t intervals, double* scra
 
 constexpr int TABLE_BLOCK = 256;
 
-// One workgroup per table entry (DeflectionTables.swift:215-233).
+// One workgroup per table entry (DeflectionTables.swift:57-64).
 __global__ __launch_bounds__(TABLE_BLOCK) void sweep_table_kernel(float* out, uint32_t n, double radius, double low,
                                                                   double high, uint32_t intervals, int kind) {
     __shared__ double scratch[TABLE_BLOCK];
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(TABLE_BLOCK) void far_field_kernel(double* b_out, d
     }
 }
 
-// ---- Kerr in double precision (KerrGeodesic.swift:322-514) ----------------
+// ---- Kerr in double precision (KerrGeodesic.swift:52-244) ----------------
 
 struct KState {
     double r, theta, phi, pr, ptheta;
@@ -156,7 +156,7 @@ GARG_DEV void kd_step(const KParams& k, KState& s, double h) {
     s.ptheta += (k1.ptheta + k2.ptheta * 2.0 + k3.ptheta * 2.0 + k4.ptheta) * w;
 }
 
-// Static observer launch in double precision (KerrGeodesic.swift:398-438).
+// Static observer launch in double precision (KerrGeodesic.swift:128-168).
 GARG_DEV void kd_launch(double a, const double o[3], const double dir[3], KState& s, KParams& k) {
     double a2 = a * a;
     double rho2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
@@ -189,7 +189,7 @@ GARG_DEV void kd_launch(double a, const double o[3], const double dir[3], KState
 }
 
 // 1 when the ray is captured under the oracle's step policy
-// (KerrGeodesic.swift:484-514), 0 when it escapes or runs out of steps.
+// (KerrGeodesic.swift:214-244), 0 when it escapes or runs out of steps.
 GARG_DEV int kd_captured(const KParams& k, KState s, uint32_t step_cap, double escape) {
     double rplus = 1.0 + sqrt(1.0 - k.a * k.a);
     double capture = rplus + KERR_CAPTURE_MARGIN;

@@ -209,7 +209,7 @@ const char* preset_name(Preset p) {
     return "default";
 }
 
-// Presets.swift:166-179, 219-241.
+// Presets.swift:18-31, 72-93.
 Configuration Configuration::make(const Options& o) {
     Configuration c;
     c.preset = o.preset;
@@ -225,7 +225,7 @@ Configuration Configuration::make(const Options& o) {
     return c;
 }
 
-// Presets.swift:196-216 with this build's layouts: a 32-byte particle, a
+// Presets.swift:48-68 with this build's layouts: a 32-byte particle, a
 // 24-byte sum and a 16-byte resolved voxel, 8 + 8 + 4 bytes per target
 // pixel, and the bake slabs.
 uint64_t Configuration::bytes() const {
@@ -260,7 +260,7 @@ void OrbitCamera::basis(float p[3], float r[3], float u[3], float f[3]) const {
     p[2] = distance * std::sin(elevation);
     float len = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
     for (int i = 0; i < 3; ++i) f[i] = -p[i] / len;
-    // right = normalize(forward x z), up = right x forward (Camera.swift:283-285).
+    // right = normalize(forward x z), up = right x forward (Camera.swift:31-33).
     float rx = f[1], ry = -f[0];
     float rl = std::sqrt(rx * rx + ry * ry);
     r[0] = rx / rl;
@@ -271,7 +271,7 @@ void OrbitCamera::basis(float p[3], float r[3], float u[3], float f[3]) const {
     u[2] = r[0] * f[1] - r[1] * f[0];
 }
 
-// Camera.swift:257-265.
+// Camera.swift:5-13.
 OrbitCamera camera_from(const Options& o) {
     OrbitCamera c;
     if (o.cam_distance) {

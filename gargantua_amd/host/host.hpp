@@ -89,7 +89,7 @@ std::string format_bytes(uint64_t bytes);
 // One sentence explaining a refusal, or empty when `bytes` fit the budget.
 std::string budget_refusal(uint64_t bytes, uint64_t budget, const std::string& what);
 
-// ---- camera and uniforms (Camera.swift:256-285, MarchPass.swift:61-98) ------
+// ---- camera and uniforms (Camera.swift:5-33, MarchPass.swift:61-98) ------
 struct OrbitCamera {
     float azimuth = 0.6f;
     float elevation = 0.32f;

@@ -1,4 +1,4 @@
-// The gargantua command line (main.swift, Program.swift:15-31): parse, then
+// The gargantua command line (main.swift, Program.swift:8-23): parse, then
 // dispatch. Usage errors exit 64, operational errors 2, failed validation 1.
 #include <cstdio>
 #include <string>

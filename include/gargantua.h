@@ -212,14 +212,14 @@ int garg_particles_checksum(garg_particles* particles, uint64_t* fnv1a);
  * one, on the device). */
 int garg_particles_dispatches(garg_particles* particles, uint32_t* out);
 
-/* ---- volume: Gpu/Volume.swift:14-43, SplatPass.swift:34-115 -------------- */
+/* ---- volume: Gpu/Volume.swift:14-43, SplatPass.swift:34-76 -------------- */
 
 /* half_extent_z = 0 makes the cube of half extent `half_extent`. */
 int garg_volume_create(garg_ctx* ctx, uint32_t size, float half_extent, float half_extent_z, garg_volume** out);
 int garg_volume_destroy(garg_volume* volume);
 /* Volume.encodeClear: zero the fixed point sums. */
 int garg_volume_clear(garg_volume* volume, garg_stream* stream);
-/* splatParticles (splat.metal:123-167): exact = 1 deposits into all eight
+/* splatParticles (splat.metal:44-88): exact = 1 deposits into all eight
  * corners, 0 picks one corner per particle from pcg4d(i, frame, seed).
  * frame = GARG_FRAME_AUTO takes the frame from the particle store's dispatch
  * count when the kernel runs, so a captured graph of step + splat + resolve
@@ -230,7 +230,7 @@ int garg_volume_clear(garg_volume* volume, garg_stream* stream);
  * the direct atomic path, which needs neither. */
 #define GARG_FRAME_AUTO 0xffffffffu
 int garg_splat_bin(garg_volume* volume, garg_particles* particles, garg_stream* stream, uint32_t exact, uint32_t frame);
-/* resolveVolume (splat.metal:172-199); clear_sums = 1 also zeroes the sums. */
+/* resolveVolume (splat.metal:93-120); clear_sums = 1 also zeroes the sums. */
 int garg_splat_resolve(garg_volume* volume, garg_stream* stream, uint32_t clear_sums);
 /* Synchronous reads: sums (2 u32, 3 i32, 1 u32 per voxel, z-major) and the
  * resolved fields (rgba16f emission + density, rgba16f velocity). */
@@ -338,7 +338,7 @@ uint64_t garg_bake_bytes(uint32_t width, uint32_t height, uint32_t capacity);
 /* bakeRays over rows [row0, row0 + rows) (encodeBakeChunk); row0 = 0 resets the overflow count. */
 int garg_bake_rows(garg_bake* bake, garg_stream* stream, const garg_march_uniforms* uniforms, garg_tables* tables,
                    uint32_t row0, uint32_t rows);
-/* walkRays (bake.metal:293-343) into `targets`. */
+/* walkRays (bake.metal:94-144) into `targets`. */
 int garg_bake_walk(garg_bake* bake, garg_stream* stream, const garg_march_uniforms* uniforms, garg_volume* volume,
                    garg_targets* targets);
 int garg_bake_overflow(garg_bake* bake, uint32_t* rays);
@@ -365,7 +365,7 @@ int garg_deflection_table(garg_ctx* ctx, garg_stream* stream, double r_obs, uint
  * screen azimuth psi, the angular radius of the capture boundary seen from a
  * static observer at (r_obs, inclination), found by multi-section over
  * `rounds` rounds of `rays_per_round` rays (Validate.swift:125-150 for the
- * multi-section, KerrGeodesic.swift:519-532 for the launch convention).
+ * multi-section, KerrGeodesic.swift:249-262 for the launch convention).
  * edge_out[k] = boundary aim angle, xi_out[k] = L / E there (optional). */
 int garg_shadow_boundary(garg_ctx* ctx, garg_stream* stream, double spin, double r_obs, double inclination_deg,
                          uint32_t azimuths, uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap,

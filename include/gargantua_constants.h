@@ -89,11 +89,11 @@
 
 /*
  * Values that live outside Constants.swift in the reference but fix results:
- * the inverse CDF length (Presets.swift:193), the Kerr oracle's default
- * minimum step (KerrGeodesic.swift:455; the kernel floor is 0.002,
+ * the inverse CDF length (Presets.swift:45), the Kerr oracle's default
+ * minimum step (KerrGeodesic.swift:176-197; the kernel floor is 0.002,
  * Kerr.h:186), the fixed point scales of the deposit (splat.metal:16-19),
  * the star cube map (Geodesic.h:190-215) and the sweep table integration
- * intervals (DeflectionTables.swift:181, Deflection.swift:23).
+ * intervals (DeflectionTables.swift:12, Deflection.swift:23).
  */
 #define INVERSE_CDF_COUNT 1024u
 #define INVERSE_CDF_RESOLUTION 8192u

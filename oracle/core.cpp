@@ -10,7 +10,7 @@
 
 extern "C" {
 
-// Hash.swift:110-122; pcg4d of Jarzynski and Olano 2020, JCGT 9(3), 20.
+// Hash.swift:9-21; pcg4d of Jarzynski and Olano 2020, JCGT 9(3), 20.
 void orc_pcg4d(const uint32_t in[4], uint32_t out[4]) {
     uint32_t v[4];
     for (int i = 0; i < 4; ++i) v[i] = in[i] * 1664525u + 1013904223u;
@@ -26,7 +26,7 @@ void orc_pcg4d(const uint32_t in[4], uint32_t out[4]) {
     for (int i = 0; i < 4; ++i) out[i] = v[i];
 }
 
-// Hash.swift:126-130: lane selects a stream by xor into the low seed word.
+// Hash.swift:25-29: lane selects a stream by xor into the low seed word.
 void orc_hash_draw(uint64_t seed, uint32_t index, uint32_t event, uint32_t lane, uint32_t out[4]) {
     uint32_t low = (uint32_t)seed;
     uint32_t high = (uint32_t)(seed >> 32);
@@ -34,10 +34,10 @@ void orc_hash_draw(uint64_t seed, uint32_t index, uint32_t event, uint32_t lane,
     orc_pcg4d(in, out);
 }
 
-// Hash.swift:133-135.
+// Hash.swift:32-34.
 double orc_unit(uint32_t x) { return (double)(x >> 8) / 16777216.0; }
 
-// Hash.swift:140-144, Box and Muller 1958.
+// Hash.swift:39-43, Box and Muller 1958.
 void orc_gaussian_pair(double u1, double u2, double out[2]) {
     double radius = std::sqrt(-2.0 * std::log(1.0 - u1));
     double angle = 2.0 * M_PI * u2;
@@ -56,7 +56,7 @@ uint64_t orc_fnv1a(const void* bytes, uint64_t count) {
     return hash;
 }
 
-// Potential.swift:158-205.
+// Potential.swift:16-60.
 static double potential_offset(double r) { return std::max(r - R_SCHWARZSCHILD, 1e-6); }
 
 double orc_potential_energy(double r, int potential) {
@@ -81,7 +81,7 @@ double orc_circular_energy(double r, int potential) {
 
 double orc_circular_angular_momentum(double r, int potential) { return r * orc_circular_speed(r, potential); }
 
-// ParticleDynamics.swift:237-244: gravity kick, drag, drift.
+// ParticleDynamics.swift:31-38: gravity kick, drag, drift.
 void orc_particle_step(double s[6], int potential, double dt, double alpha) {
     double r = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
     double k = -orc_potential_gradient(r, potential) / r;
@@ -92,7 +92,7 @@ void orc_particle_step(double s[6], int potential, double dt, double alpha) {
     for (int i = 0; i < 3; ++i) s[i] += s[3 + i] * dt;
 }
 
-// ParticleDynamics.swift:249-258: swept capture, escape, speed limit.
+// ParticleDynamics.swift:43-52: swept capture, escape, speed limit.
 int orc_particle_removed(const double before[3], const double after[6]) {
     double travel[3] = {after[0] - before[0], after[1] - before[1], after[2] - before[2]};
     double length_squared = std::max(travel[0] * travel[0] + travel[1] * travel[1] + travel[2] * travel[2], 1e-12);
@@ -111,7 +111,7 @@ double orc_specific_energy(const double s[6], int potential) {
     return 0.5 * v2 + orc_potential_energy(r, potential);
 }
 
-// DiskModel.swift:290-300.
+// DiskModel.swift:17-21.
 double orc_disk_shape(double r, double floor) {
     double x = R_ISCO / r;
     double torque = std::max(1.0 - std::sqrt(x), floor);
@@ -123,7 +123,7 @@ double orc_disk_temperature(double r, double peak) {
     return peak * orc_disk_shape(r, ZERO_TORQUE_FLOOR) / orc_disk_shape(peak_radius, ZERO_TORQUE_FLOOR);
 }
 
-// DiskModel.swift:304-323.
+// DiskModel.swift:31-35.
 static double mass_flux_fraction(double r) {
     if (r <= R_FEED_INNER) return 1.0;
     if (r >= R_DISK_OUTER) return 0.0;
@@ -137,7 +137,7 @@ double orc_steady_surface_density(double r) {
 
 double orc_radial_number_density(double r) { return 2.0 * M_PI * r * orc_steady_surface_density(r); }
 
-// DiskModel.swift:327-351: trapezoid cumulative, linear inversion.
+// DiskModel.swift:54-78: trapezoid cumulative, linear inversion.
 void orc_radial_inverse_cdf(int count, int resolution, double* table) {
     double r0 = R_ISCO, r1 = R_DISK_OUTER;
     double dr = (r1 - r0) / (double)resolution;
@@ -166,7 +166,7 @@ void orc_radial_inverse_cdf(int count, int resolution, double* table) {
     table[count - 1] = r1;
 }
 
-// DiskModel.swift:356-380 (generator 0); generator 1 is the annulus scene
+// DiskModel.swift:83-107 (generator 0); generator 1 is the annulus scene
 // generator of the benchmark configurations: the respawn radius law at every
 // event and a fixed Gaussian height sigma_z.
 void orc_spawn(uint32_t index, uint32_t event, uint64_t seed, const double* cdf, int cdf_count, int potential,
@@ -200,7 +200,7 @@ void orc_spawn(uint32_t index, uint32_t event, uint64_t seed, const double* cdf,
     out[5] = FEED_VELOCITY_DISPERSION * vc * g23[1];
 }
 
-// DiskEdge.swift:279-308.
+// DiskEdge.swift:28-43.
 double orc_inner_edge(const double* radii, int64_t n, double bin_width, double fit_inner, double fit_outer) {
     int count = (int)std::llround((fit_outer - fit_inner) / bin_width);
     std::vector<double> counts(count, 0.0);
@@ -224,7 +224,7 @@ double orc_inner_edge(const double* radii, int64_t n, double bin_width, double f
     return -intercept / slope;
 }
 
-// GoldenTests.swift:403-428: seed, step, respawn on removal, cylindrical radii.
+// GoldenTests.swift:39-64: seed, step, respawn on removal, cylindrical radii.
 void orc_disk_run(uint32_t count, uint64_t seed, int substeps, int threads, double* radii, uint64_t* respawns_out) {
     std::vector<double> cdf(INVERSE_CDF_COUNT);
     orc_radial_inverse_cdf((int)INVERSE_CDF_COUNT, (int)INVERSE_CDF_RESOLUTION, cdf.data());
@@ -251,7 +251,7 @@ void orc_disk_run(uint32_t count, uint64_t seed, int substeps, int threads, doub
     if (respawns_out) *respawns_out = total;
 }
 
-// Blackbody.swift:184-264: Planck spectrum, Wyman, Sloan and Shirley 2013
+// Blackbody.swift:13-82: Planck spectrum, Wyman, Sloan and Shirley 2013
 // multi-lobe CIE fit, IEC 61966-2-1 linear sRGB.
 static double planck(double nm, double temperature) {
     const double c2 = 1.438776877e-2;

@@ -303,7 +303,7 @@ double orc_strong_deflection(double b) {
 
 // ---- KerrGeodesic.swift ---------------------------------------------------
 
-// KerrGeodesic.swift:454-466.
+// KerrGeodesic.swift:176-197.
 void orc_kerr_settings_default(orc_kerr_settings* s, int64_t step_cap) {
     s->step_factor = STEP_RADIUS_FACTOR;
     s->step_min = 0.005;
@@ -321,7 +321,7 @@ static double ksigma(double a, double r, double theta) {
     return r * r + a * a * c * c;
 }
 
-// KerrGeodesic.swift:322-329.
+// KerrGeodesic.swift:52-59.
 static void potentials(double a, const orc_kerr_state& s, double& p, double& g) {
     double e = s.energy, l = s.angular_momentum;
     p = (s.r * s.r + a * a) * e - a * l;
@@ -342,7 +342,7 @@ double orc_kerr_carter(double a, const orc_kerr_state* s) {
     return s->ptheta * s->ptheta + c * c * term;
 }
 
-// KerrGeodesic.swift:347-375.
+// KerrGeodesic.swift:77-105.
 static orc_kerr_state kderivative(double a, const orc_kerr_state& s) {
     double e = s.energy, l = s.angular_momentum;
     double r = s.r, theta = s.theta;
@@ -377,7 +377,7 @@ static orc_kerr_state kaxpy(const orc_kerr_state& s, const orc_kerr_state& k, do
     return o;
 }
 
-// KerrGeodesic.swift:377-383.
+// KerrGeodesic.swift:107-113.
 void orc_kerr_step(double a, orc_kerr_state* s, double h) {
     orc_kerr_state k1 = kderivative(a, *s);
     orc_kerr_state k2 = kderivative(a, kaxpy(*s, k1, 0.5 * h));
@@ -391,7 +391,7 @@ void orc_kerr_step(double a, orc_kerr_state* s, double h) {
     s->ptheta += (k1.ptheta + k2.ptheta * 2.0 + k3.ptheta * 2.0 + k4.ptheta) * w;
 }
 
-// KerrGeodesic.swift:386-391.
+// KerrGeodesic.swift:116-121.
 double orc_kerr_radius_of(double a, const double point[3]) {
     double a2 = a * a;
     double rho2 = point[0] * point[0] + point[1] * point[1] + point[2] * point[2];
@@ -399,7 +399,7 @@ double orc_kerr_radius_of(double a, const double point[3]) {
     return std::sqrt(half + std::sqrt(half * half + a2 * point[2] * point[2]));
 }
 
-// KerrGeodesic.swift:398-438: static observer tetrad, local energy one.
+// KerrGeodesic.swift:128-168: static observer tetrad, local energy one.
 void orc_kerr_launch(double a, const double origin[3], const double direction[3], orc_kerr_state* out) {
     double r = orc_kerr_radius_of(a, origin);
     double rho = std::sqrt(origin[0] * origin[0] + origin[1] * origin[1] + origin[2] * origin[2]);
@@ -430,7 +430,7 @@ void orc_kerr_launch(double a, const double origin[3], const double direction[3]
     out->ptheta = gthth * (nth / std::sqrt(gthth));
 }
 
-// KerrGeodesic.swift:440-444.
+// KerrGeodesic.swift:170-174.
 void orc_kerr_position(double a, const orc_kerr_state* s, double out[3]) {
     double radial = std::sqrt(s->r * s->r + a * a);
     double st = std::sin(s->theta);
@@ -439,7 +439,7 @@ void orc_kerr_position(double a, const orc_kerr_state* s, double out[3]) {
     out[2] = s->r * std::cos(s->theta);
 }
 
-// The direction convention of the kernel (Kerr.h:396-411), which the Swift
+// The direction convention of the kernel (Kerr.h:158-173), which the Swift
 // oracle shares but does not expose, restated in FP64.
 void orc_kerr_direction(double a, const orc_kerr_state* s, double out[3]) {
     double delta = kdelta(a, s->r), sigma = ksigma(a, s->r, s->theta);
@@ -458,7 +458,7 @@ void orc_kerr_direction(double a, const orc_kerr_state* s, double out[3]) {
     orc::store(len > 0 ? local / len : r_hat, out);
 }
 
-// KerrGeodesic.swift:484-491.
+// KerrGeodesic.swift:214-221.
 double orc_kerr_step_length(double a, const orc_kerr_state* s, const orc_kerr_settings* settings) {
     double above = s->r - orc_kerr_outer_horizon(a);
     double near_h = 0.2 * above;
@@ -467,7 +467,7 @@ double orc_kerr_step_length(double a, const orc_kerr_state* s, const orc_kerr_se
                     settings->step_max);
 }
 
-// KerrGeodesic.swift:493-514.
+// KerrGeodesic.swift:223-244.
 void orc_kerr_integrate(double a, const orc_kerr_state* launch, const orc_kerr_settings* settings, orc_kerr_result* out) {
     orc_kerr_state s = *launch;
     double carter0 = orc_kerr_carter(a, launch);
@@ -497,7 +497,7 @@ void orc_kerr_integrate(double a, const orc_kerr_state* launch, const orc_kerr_s
     out->drift = drift;
 }
 
-// KerrGeodesic.swift:519-532.
+// KerrGeodesic.swift:249-262.
 double orc_kerr_bisect_critical_xi(double a, double radius, double side, const orc_kerr_settings* settings, int iterations) {
     double low = 0.0, high = 12.0 / radius;
     double origin[3] = {radius, 0.0, 0.0};
@@ -517,7 +517,7 @@ double orc_kerr_bisect_critical_xi(double a, double radius, double side, const o
     return boundary.angular_momentum / boundary.energy;
 }
 
-// KerrGeodesic.swift:537-551, Bardeen 1973.
+// KerrGeodesic.swift:267-273, Bardeen 1973.
 double orc_kerr_critical_xi(double a, double rc) {
     return (rc * rc * (3.0 - rc) - a * a * (rc + 1.0)) / (a * (rc - 1.0));
 }

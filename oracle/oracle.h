@@ -55,7 +55,7 @@ void orc_radial_inverse_cdf(int count, int resolution, double* out);
 void orc_spawn(uint32_t index, uint32_t event, uint64_t seed, const double* cdf, int cdf_count,
                int potential, int generator, double sigma_z, double out[6]);
 double orc_inner_edge(const double* radii, int64_t n, double bin_width, double fit_inner, double fit_outer);
-/* Full FP64 disk run as the isco golden does it (GoldenTests.swift:403-428):
+/* Full FP64 disk run as the isco golden does it (GoldenTests.swift:39-64):
  * returns cylindrical radii after `substeps` substeps. */
 void orc_disk_run(uint32_t count, uint64_t seed, int substeps, int threads, double* radii_out, uint64_t* respawns_out);
 
@@ -195,7 +195,7 @@ typedef struct {
 } orc_camera;
 
 void orc_camera_direction(const orc_camera* c, int px, int py, int width, int height, double out[3]);
-/* LensedRender.trace (LensedRender.swift:110-142): gas only, no stars. */
+/* LensedRender.trace (LensedRender.swift:46-78): gas only, no stars. */
 void orc_trace(const double origin[3], const double direction[3], const orc_field* field,
                const orc_settings* settings, double opacity, double slab, int redshift, double out[3]);
 void orc_render(int width, int height, const orc_camera* camera, const orc_field* field,

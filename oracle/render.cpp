@@ -47,7 +47,7 @@ void orc_field_sample(const orc_field* field, const double position[3], double e
     }
 }
 
-// LensedRender.swift:158-173.
+// LensedRender.swift:94-109.
 double orc_luminance(const double rgb[3]) { return 0.2126 * rgb[0] + 0.7152 * rgb[1] + 0.0722 * rgb[2]; }
 
 double orc_side_ratio(const double* luminance, int width, int height) {
@@ -75,7 +75,7 @@ static double step_length(const orc_settings* g, double r, double z) {
     return std::min(std::max(g->step_factor * r, g->step_min), g->step_max);
 }
 
-// LensedRender.swift:110-142.
+// LensedRender.swift:46-78.
 void orc_trace(const double origin[3], const double direction[3], const orc_field* field, const orc_settings* settings,
                double opacity, double slab, int redshift, double out[3]) {
     orc_ray ray;
@@ -119,7 +119,7 @@ void orc_trace(const double origin[3], const double direction[3], const orc_fiel
     orc::store(color, out);
 }
 
-// LensedRender.swift:145-155, multi-threaded over rows.
+// LensedRender.swift:81-91, multi-threaded over rows.
 void orc_render(int width, int height, const orc_camera* camera, const orc_field* field, const orc_settings* settings,
                 double opacity, int redshift, int threads, double* out_rgb) {
     orc::parallel_dynamic(height, threads, 1, [&](int64_t y0, int64_t y1) {

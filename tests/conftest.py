@@ -23,7 +23,7 @@ def load_golden(name: str) -> dict:
 
 
 def golden_passes(g: dict, measured: float) -> bool:
-    """Golden.passes (Oracle/Golden.swift:354-360)."""
+    """Golden.passes (Oracle/Golden.swift:45-51)."""
     if g["kind"] == "relative":
         return abs(measured - g["expected"]) / abs(g["expected"]) <= g["tolerance"]
     if g["kind"] == "absolute":

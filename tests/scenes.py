@@ -1,5 +1,5 @@
 """Scene builders shared by the GPU tests: the reference's probe launch sets
-(Validation/Probes.swift:448-469), cameras, and oracle parameter blocks
+(Validation/Probes.swift:45-66), cameras, and oracle parameter blocks
 built from the exact float32 uniforms the GPU receives."""
 from __future__ import annotations
 
@@ -85,7 +85,7 @@ def oracle_params(u: api.MarchUniforms, tables: tuple | None = None) -> O.MarchP
 
 
 def oracle_tables(camera_radius: float, sphere_radius: float = 30.0):
-    """The three sweep tables in FP64 (DeflectionTables.swift:198-233)."""
+    """The three sweep tables in FP64 (DeflectionTables.swift:29-64)."""
     sphere = O.sweep_table(sphere_radius, 1024, 2048)
     camera = O.sweep_table(camera_radius, 1024, 256)
     sky = O.sky_table(camera_radius, sphere_radius - 1.0, 1024, 256)

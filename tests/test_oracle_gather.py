@@ -5,7 +5,7 @@ march.metal:81-102 (Schwarzschild) and :151-170 (Kerr). Three independent
 checks:
 
 * at spin 0 with no sphere jump and no stars the restatement is the
-  LensedRender.trace restatement (LensedRender.swift:46-82), which the
+  LensedRender.trace restatement (LensedRender.swift:46-78), which the
   reference's own LensedRender properties pin (mirror symmetry, beaming), on
   the reference's SyntheticDisk fixture (LensedRenderTests.swift:7-53);
 * the Kerr loop as the spin goes to zero is the Schwarzschild loop, pixel by

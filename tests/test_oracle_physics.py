@@ -100,7 +100,7 @@ def test_tunnelling_grazing_and_runaway():
     assert O.lib.orc_particle_removed(d3(9.0, 0.0, 0.0), d6(10.0, 0.0, 0.0, 40.0, 0.0, 0.0))
 
 
-# ---- disk model and edge (OracleTests.swift:106-157, GoldenTests.swift:430-441) --
+# ---- disk model and edge (OracleTests.swift:106-157, GoldenTests.swift:66-77) --
 
 def test_temperature_peaks_at_49_over_36_isco():
     peak = O.lib.orc_disk_temperature(6.0 * 49.0 / 36.0, 8000.0)
@@ -155,7 +155,7 @@ def test_edge_recovers_analytic_profile():
 
 
 def test_isco_golden_in_the_oracle():
-    """GoldenTests.swift:403-428: 24k particles, seed 11, 1500 substeps."""
+    """GoldenTests.swift:39-64: 24k particles, seed 11, 1500 substeps."""
     g = load_golden("isco")
     n = 24_000
     radii = np.zeros(n)
