@@ -262,7 +262,7 @@ def main():
                    "step_cap": 4096, "observer_radius": 50.0, "inclination_deg": 60.0, "fov_deg": 60.0,
                    "particles": 1_000_000, "volume": 128, "volume_half_extent_z": 3.0, "settle_substeps": 600,
                    "rays": rays, "steps_per_frame": steps_per_frame, "compensated": True,
-                   "form": "polar (Boyer-Lindquist fast steps while sin(theta) >= 0.25, Kerr-Schild after)",
+                   "form": "polar (Boyer-Lindquist fast steps until sin(theta) < 0.25 or a plunge inside r+ + 0.5, Kerr-Schild after)",
                    "batch": {"block": cfg.block, "waves_per_simd": cfg.waves_per_simd,
                              "compaction_period": cfg.compaction_period, "refill_threshold": cfg.refill_threshold,
                              "kerr_schild": cfg.kerr_schild}},

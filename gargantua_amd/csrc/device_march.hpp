@@ -1327,18 +1327,32 @@ struct KerrSchildMarch {
 };
 
 // ---------------------------------------------------------------------------
-// Kerr with Kerr-Schild coordinates only near the spin axis. Boyer-Lindquist
-// coordinates are singular on the axis, and every ray whose Boyer-Lindquist
-// march breaks the drift budget passes close to it (on the metric's 1080p
-// frame at a = 0.9 every such ray dips below sin(theta) = 0.2); away from it
-// the fast Boyer-Lindquist step is the cheaper one. A ray therefore marches
-// in the fast Boyer-Lindquist form until sin(theta) first falls below
-// GARG_POLAR_ENTER and in Kerr-Schild coordinates from there to its end.
-// Both forms take the same step length there (the pole factor is 1 above
-// sin(theta) = 0.08); the launch's Carter constant stays the drift
+// Kerr with Kerr-Schild coordinates only near the spin axis and on fast
+// plunges into the horizon. Boyer-Lindquist coordinates are singular on the
+// axis, and every ray whose Boyer-Lindquist march breaks the drift budget on
+// the 1080p frame at a = 0.9 passes close to it (every such ray dips below
+// sin(theta) = 0.2). They are singular on the horizon too: near it the step
+// is at most 0.2 (r - r+), so a ray falling faster than dr/dlambda = -5
+// (retrograde plungers at a >= 0.93 fall at -5 to -7) takes RK4 substeps
+// onto Delta ~ 0, after which its FP32 state is non-finite and it leaves as
+// "escaped" one step before the Kerr-Schild march (regular across the
+// future horizon) captures it. Rays that graze the horizon and may still
+// turn round (prograde, |dr/dlambda| < 1) keep their Boyer-Lindquist steps,
+// which hold their drift better than a late switch: handing every ray
+// inside r+ + 0.5 to Kerr-Schild raises the frame's largest escaping drift
+// from 6.4e-6 to 9.7e-6 and puts 255 rays of the mix's high-spin stills
+// over budget. A ray therefore marches in the fast Boyer-Lindquist form
+// until sin(theta) first falls below GARG_POLAR_ENTER, or it is inside
+// r+ + GARG_HORIZON_ENTER falling faster than GARG_PLUNGE_RATE, and in
+// Kerr-Schild coordinates from there to its end. On the a = 0.9 frame 1.1%
+// of the steps are below that sin(theta). Both forms take the same step
+// length (the pole factor is 1 above sin(theta) = 0.08; the horizon factor
+// depends on r alone); the launch's Carter constant stays the drift
 // reference and the gather carries across; the coordinate change runs in
-// double precision, once per polar ray.
+// double precision, once per ray that switches.
 #define GARG_POLAR_ENTER 0.25f
+#define GARG_HORIZON_ENTER 0.5f
+#define GARG_PLUNGE_RATE 2.0f
 
 // Boyer-Lindquist (r, theta, phi, p_r, p_theta; E, L) to Kerr-Schild
 // (x, y, z, p_x, p_y, p_z), with the Kerr-Schild azimuth equal to phi here:
@@ -1476,7 +1490,7 @@ struct KerrHybridMarch {
         m.begin(spin, launched);
         polar = false;
         pack(m);
-        if (m.e.st < GARG_POLAR_ENTER) to_polar(m);
+        if (m.e.st < GARG_POLAR_ENTER || (m.r < m.k.rplus + GARG_HORIZON_ENTER && m.e.dr < -GARG_PLUNGE_RATE)) to_polar(m);
     }
 
     GARG_DEV bool done() const { return ks.done; }
@@ -1486,8 +1500,8 @@ struct KerrHybridMarch {
     }
 
     // Two phases, so that only one form's state is live at a time: the
-    // Boyer-Lindquist steps until the ray ends or reaches the axis, then the
-    // Kerr-Schild steps for the rest of the budget.
+    // Boyer-Lindquist steps until the ray ends, nears the axis or plunges
+    // into the horizon, then the Kerr-Schild steps for the rest of the budget.
     template <bool UNIFORM, class Tick = NoTick>
     GARG_DEV void run_loop(KerrLoop l, uint32_t budget, Tick tick = Tick{}) {
         if constexpr (UNIFORM) l = wave_uniform(l);
@@ -1498,7 +1512,7 @@ struct KerrHybridMarch {
             for (; n < budget && !m.done; ++n) {
                 tick(n);
                 m.step(l);
-                if (!m.done && m.e.st < GARG_POLAR_ENTER) {
+                if (!m.done && (m.e.st < GARG_POLAR_ENTER || (m.r < l.rp + GARG_HORIZON_ENTER && m.e.dr < -GARG_PLUNGE_RATE))) {
                     axis = true;
                     ++n;
                     break;

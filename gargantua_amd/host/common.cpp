@@ -34,7 +34,7 @@ void check(int status, const char* what) {
 
 const char* kUsage =
     "usage: gargantua [--preset small|default|max] [--particles N] [--volume 96|128|192|256] [--seed S] "
-    "[--strategy march|bake] [--fps-cap 30|60] [--no-upscale] [--spin A] [--kerr-schild] [--camera az,el,dist] "
+    "[--strategy march|bake] [--fps-cap 30|60] [--no-upscale] [--spin A] [--form fast|kerr-schild|polar] [--camera az,el,dist] "
     "[--still out.png --width W --height H [--sequence N]] | bench [--sweep | --sweep-batch] | validate | serve [--rate R] "
     "[--seconds T] [--deadline D] [--frames F] [--tick-rays N] [--policy srpt|fifo] [--bake-share F] [--max-wait S] "
     "[--verify N] [--report PATH] | version | constants";
@@ -140,7 +140,13 @@ std::string parse_options(const std::vector<std::string>& args, Options& o) {
             if (!parse_double(text, s) || s < 0 || s > 0.998) return "spin must be between 0 and 0.998.";
             o.spin = s;
         } else if (a == "--kerr-schild") {
-            o.kerr_schild = true;
+            o.form = 1;
+        } else if (a == "--form") {
+            if (!(err = value(a, text)).empty()) return err;
+            if (text == "fast") o.form = 0;
+            else if (text == "kerr-schild") o.form = 1;
+            else if (text == "polar") o.form = 2;
+            else return "form must be fast, kerr-schild or polar.";
         } else if (a == "--camera") {
             if (!(err = value(a, text)).empty()) return err;
             double p[3];
@@ -207,6 +213,10 @@ const char* preset_name(Preset p) {
         case Preset::max: return "max";
     }
     return "default";
+}
+
+const char* form_name(unsigned form) {
+    return form == 0 ? "fast" : form == 1 ? "kerr-schild" : "polar";
 }
 
 // Presets.swift:18-31, 72-93.

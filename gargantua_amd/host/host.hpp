@@ -43,9 +43,13 @@ struct Options {
     std::optional<int> sequence;
     std::optional<double> cam_azimuth, cam_elevation, cam_distance;
     double spin = 0.0;
-    // Spinning rays in Kerr-Schild coordinates (regular on the spin axis)
-    // instead of the reference's Boyer-Lindquist march.
-    bool kerr_schild = false;
+    // How spinning rays step (garg_batch_config.kerr_schild): 0 Boyer-Lindquist
+    // fast steps throughout (the reference's coordinates; its E/L drift blows up
+    // near the spin axis), 1 Kerr-Schild coordinates throughout (regular on the
+    // axis), 2 the polar form (Boyer-Lindquist until a ray's sin(theta) drops
+    // below 0.25, Kerr-Schild from there), which holds E/L drift under 1e-5 on
+    // escaping rays at the cost of the fast form plus a few percent.
+    unsigned form = 2;
     // serve: arrival rate, duration, scale-down of the workload for a bounded run
     double serve_rate = 20.0;
     double serve_seconds = 300.0;
@@ -85,6 +89,7 @@ struct Configuration {
     uint64_t bytes() const;
 };
 const char* preset_name(Preset p);
+const char* form_name(unsigned form);  // the --form spelling of Options::form
 std::string format_bytes(uint64_t bytes);
 // One sentence explaining a refusal, or empty when `bytes` fit the budget.
 std::string budget_refusal(uint64_t bytes, uint64_t budget, const std::string& what);

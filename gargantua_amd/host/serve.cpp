@@ -189,7 +189,7 @@ void run_serve(const Options& o) {
 
     garg_batch_config cfg;
     check(garg_batch_config_default(s.ctx, &cfg), "reading the batch configuration");
-    cfg.kerr_schild = o.kerr_schild ? 1u : 0u;
+    cfg.kerr_schild = o.form;
     const uint64_t kTickRays = (uint64_t)o.serve_tick_rays;  // rays marched per tick across all requests
     size_t next_arrival = 0;
     std::deque<int> pending;
@@ -459,7 +459,7 @@ void run_serve(const Options& o) {
              "\"latency_p50_s\": %.4f, \"latency_p99_s\": %.4f, \"latency_max_s\": %.4f, \"steps\": %llu, "
              "\"steps_per_s\": %.4e, \"gpu_busy_fraction\": %.3f, \"exhausted_rays\": %llu, \"truncated_rays\": %llu, "
              "\"verified\": %llu, \"verified_baked\": %llu, \"mismatched\": %llu, \"rate\": %.3f, "
-             "\"arrival_seconds\": %.3f, \"frames\": %d, \"policy\": \"%s\", \"kerr_schild\": %s}",
+             "\"arrival_seconds\": %.3f, \"frames\": %d, \"policy\": \"%s\", \"form\": \"%s\"}",
              all.size(), completed, unfinished, (unsigned long long)refused, (unsigned long long)shed, o.serve_max_wait,
              (unsigned long long)baked, wall, (unsigned long long)ticks, pct(0.5),
              pct(0.99),
@@ -467,7 +467,7 @@ void run_serve(const Options& o) {
              gpu_busy / wall, (unsigned long long)exhausted, (unsigned long long)truncated,
              (unsigned long long)verified, (unsigned long long)verified_bakes, (unsigned long long)mismatched,
              o.serve_rate, o.serve_seconds,
-             o.serve_sequence_frames, o.serve_policy.c_str(), o.kerr_schild ? "true" : "false");
+             o.serve_sequence_frames, o.serve_policy.c_str(), form_name(o.form));
     line(buf);
     if (!o.serve_report.empty()) {
         FILE* f = fopen(o.serve_report.c_str(), "w");

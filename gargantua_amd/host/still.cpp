@@ -17,7 +17,7 @@ std::string sibling(const std::string& path, const std::string& name) {
     return slash == std::string::npos ? name : path.substr(0, slash + 1) + name;
 }
 
-void march(garg_ctx* ctx, garg_march_uniforms& u, Scene& s, garg_targets* t, bool kerr_schild) {
+void march(garg_ctx* ctx, garg_march_uniforms& u, Scene& s, garg_targets* t, unsigned form) {
     garg_march_request r{};
     r.uniforms = u;
     r.targets = t;
@@ -25,7 +25,7 @@ void march(garg_ctx* ctx, garg_march_uniforms& u, Scene& s, garg_targets* t, boo
     r.volume = s.volume;
     garg_batch_config cfg;
     check(garg_batch_config_default(ctx, &cfg), "reading the batch configuration");
-    cfg.kerr_schild = kerr_schild ? 1u : 0u;
+    cfg.kerr_schild = form;
     check(garg_march_batch(ctx, nullptr, &r, 1, &cfg, nullptr), "marching");
 }
 
@@ -54,7 +54,7 @@ void run_still(const Options& o) {
     garg_targets* full = nullptr;
     check(garg_targets_create(s.ctx, (uint32_t)width, (uint32_t)height, &full), "allocating the still");
     line("rendering " + std::to_string(width) + " by " + std::to_string(height));
-    march(s.ctx, u, s, full, o.kerr_schild);
+    march(s.ctx, u, s, full, o.form);
     std::vector<uint8_t> rgba((size_t)width * height * 4);
     if (o.sequence) {
         const int sw = 960, sh = 540;
@@ -68,7 +68,7 @@ void run_still(const Options& o) {
             garg_march_uniforms fu = make_uniforms(camera, sw, sh, Policy::interactive, true, (uint32_t)c.seed, true,
                                                    c.spin, false);
             if (c.spin == 0.0) check(garg_tables_apply(s.tables, &fu), "applying the sweep tables");
-            march(s.ctx, fu, s, seq, o.kerr_schild);
+            march(s.ctx, fu, s, seq, o.form);
             check(garg_tonemap(seq, nullptr, GARG_F(EXPOSURE_DEFAULT), 0u, 1.0f, frame_rgba.data()), "tone mapping");
             char name[32];
             snprintf(name, sizeof name, "seq-%03d.png", f);

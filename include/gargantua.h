@@ -56,11 +56,13 @@ enum { GARG_GENERATOR_DISK = 0, GARG_GENERATOR_ANNULUS = 1 };
  * future work for its polar seam (docs/lab/07-kerr.md:33-45). It takes the
  * same launch, step policy (without the pole factor), gas and sky as the
  * Boyer-Lindquist forms; at spin 0 it is the FAST Schwarzschild form.
- * POLAR marches a Kerr ray in the FAST form while sin(theta) >= 0.25 and in
- * the KERR_SCHILD form from there until sin(theta) > 0.3, converting the
- * state in double precision at each switch: regular on the axis at about the
- * cost of the FAST form, since only rays that pass near the axis leave it;
- * at spin 0 it is the FAST Schwarzschild form.
+ * POLAR marches a Kerr ray in the FAST form until sin(theta) first falls
+ * below 0.25, or the ray is inside r+ + 0.5 falling faster than
+ * dr/dlambda = -2, and in the KERR_SCHILD form from there to its end,
+ * converting the state once, in double precision: regular on the axis and
+ * across the horizon at about the cost of the FAST form, since only rays
+ * that pass near the axis or plunge leave it; at spin 0 it is the FAST
+ * Schwarzschild form.
  * The batched march runs the FAST form, or for spinning requests the
  * KERR_SCHILD or POLAR form when garg_batch_config.kerr_schild asks for it. */
 enum {
@@ -312,7 +314,7 @@ typedef struct {
                                    the end before it takes the next 64 */
     uint32_t kerr_schild;       /* spinning requests: 0 = the Boyer-Lindquist FAST form,
                                    1 = the Kerr-Schild form (regular on the spin axis),
-                                   2 = the POLAR form (Kerr-Schild near the axis only) */
+                                   2 = the POLAR form (Kerr-Schild near the axis and on plunges only) */
 } garg_batch_config;
 
 /* Rays of a request in tile order: 8x8 pixel tiles row-major, 64 rays each. */

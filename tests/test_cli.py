@@ -34,6 +34,7 @@ def run(*args, timeout=600, cwd=ROOT, env=None):
     (["serve", "--bake-share", "1.5"], 64, "bake-share needs a number between 0 and 1."),
     (["serve", "--max-wait", "0"], 64, "max-wait needs a positive number."),
     (["--kerr-schild", "--spin", "1.5"], 64, "spin must be between 0 and 0.998."),
+    (["--form", "boyer-lindquist"], 64, "form must be fast, kerr-schild or polar."),
     ([], 2, "macOS only"),
 ])
 def test_option_grammar_and_exit_codes(args, code, sentence):
@@ -96,23 +97,24 @@ def test_validate_passes_every_judged_golden():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tick_rays,policy,kerr_schild", [(4 << 20, "srpt", False), (50_000, "srpt", False),
-                                                     (50_000, "fifo", False), (50_000, "srpt", True)])
-def test_serve_results_equal_alone_and_in_the_mix(tmp_path, tick_rays, policy, kerr_schild):
+@pytest.mark.parametrize("tick_rays,policy,form", [(4 << 20, "srpt", "polar"), (50_000, "srpt", "polar"),
+                                              (50_000, "fifo", "polar"), (50_000, "srpt", "kerr-schild"),
+                                              (50_000, "srpt", "fast")])
+def test_serve_results_equal_alone_and_in_the_mix(tmp_path, tick_rays, policy, form):
     """Every request re-rendered alone equals its render in the mix; a small
     per-tick ray budget splits frames over many ticks and leaves requests
     waiting for budget, which each policy hands out in its own order; the
-    spinning requests also in the Kerr-Schild form."""
+    spinning requests in each of the three forms (polar is the default)."""
     report = tmp_path / "serve.json"
     r = run("serve", "--preset", "small", "--rate", "6", "--seconds", "3", "--frames", "3", "--verify", "1",
             "--tick-rays", str(tick_rays), "--policy", policy, "--report", str(report),
-            *(["--kerr-schild"] if kerr_schild else []))
+            *(["--form", form] if form != "polar" else []))
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     rep = json.loads(report.read_text())
     assert rep["completed"] > 0 and rep["verified"] > 0 and rep["mismatched"] == 0
     assert rep["latency_p99_s"] >= rep["latency_p50_s"] > 0
-    assert rep["policy"] == policy and rep["kerr_schild"] == kerr_schild
+    assert rep["policy"] == policy and rep["form"] == form
 
 
 @pytest.mark.gpu
