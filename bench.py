@@ -535,15 +535,22 @@ def mixed_leg(ctx, api, vol, args):
     for r in reqs:
         r["targets"].reset_counters()
     ptotals = ctx.march_batch(reqs, polar)
-    over = 0
+    # Escaping or exhausted rays over their request's drift budget, split by
+    # policy: stills (4096 steps, compensated, 1e-5) and interactive requests
+    # (256 steps, uncompensated, 1e-4, where most rays end exhausted).
+    over = {"still": 0, "interactive": 0}
+    non_finite = 0
     for r in reqs:
-        debug = r["targets"].read()[1]
+        u = r["uniforms"]
+        debug = r["targets"].read()[1].astype(np.float64)
         kept = debug[..., 2] != api.RAY_CAPTURED
-        over += int((debug[..., 0][kept] > r["uniforms"].drift_budget).sum())
+        over["still" if u.compensated else "interactive"] += int((debug[..., 0][kept] > u.drift_budget).sum())
+        non_finite += int((~np.isfinite(debug[..., 0][kept])).sum())
     polar_ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, reqs, polar), 1, 3)
     out["polar"] = {"batched_ms": polar_ms / 3, "steps": int(ptotals.steps),
                     "batched_steps_per_s": int(ptotals.steps) / (polar_ms / 3 * 1e-3),
-                    "rays_over_budget_escaped_or_exhausted": over}
+                    "rays_over_budget_escaped_or_exhausted": over["still"] + over["interactive"],
+                    "rays_over_budget_by_policy": over, "rays_non_finite_drift": non_finite}
     for name, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST),
                              ("plain_polar", api.INTEGRATOR_POLAR)):
         def run_all():
