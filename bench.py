@@ -589,6 +589,9 @@ def pass_leg(ctx, api, particles, vol):
     # 50 consecutive dispatches in one call: one launch with the particles
     # in registers (the settle and any multi-dispatch step take this path).
     fused_ms = timed(lambda: particles.step(50), 4) / 50
+    from gargantua_amd import autotune
+    brick = autotune.best_brick(ctx, "splat_1M_128")
+    vol.brick = brick
     bin_ms = timed(lambda: vol.splat(particles, exact=True), 10)
     # The direct exact deposit (one global atomic per corner sum): inside a
     # graph capture the exact bin takes that path, so time its replays.
@@ -613,7 +616,7 @@ def pass_leg(ctx, api, particles, vol):
             # Exact bin: 8 corners x 6 integer sums per particle. Sorted and
             # deposited per voxel brick, it issues no global atomics, so its
             # corner-sum rate is compared with what direct atomics reach.
-            "splat_bin_exact": {"ms": bin_ms, "deposits_per_s": n / (bin_ms * 1e-3),
+            "splat_bin_exact": {"ms": bin_ms, "brick": list(brick), "deposits_per_s": n / (bin_ms * 1e-3),
                                 "corner_sums_per_s": 8 * 6 * n / (bin_ms * 1e-3),
                                 "direct_atomic_ceiling_per_s": ceiling,
                                 "vs_direct_atomic_ceiling": 8 * 6 * n / (bin_ms * 1e-3) / ceiling,

@@ -115,6 +115,8 @@ SIGNATURES = {
     "garg_volume_create": (C.c_int, [vp, u32, f32, f32, P(vp)]),
     "garg_volume_destroy": (C.c_int, [vp]),
     "garg_volume_clear": (C.c_int, [vp, vp]),
+    "garg_volume_set_splat_brick": (C.c_int, [vp, u32, u32, u32]),
+    "garg_volume_splat_brick": (C.c_int, [vp, P(u32)]),
     "garg_splat_bin": (C.c_int, [vp, vp, vp, u32, u32]),
     "garg_splat_resolve": (C.c_int, [vp, vp, u32]),
     "garg_volume_read_sums": (C.c_int, [vp, vp, vp, vp]),
@@ -460,6 +462,19 @@ class Volume:
 
     def clear(self, stream=None):
         _check(self.ctx.lib.garg_volume_clear(self.h, stream))
+
+    # The exact deposit's LDS brick shapes (garg_volume_set_splat_brick); the first is the default.
+    BRICKS = ((16, 8, 4), (8, 8, 4), (8, 8, 8), (16, 16, 4), (32, 8, 4), (16, 8, 8), (32, 16, 4), (16, 16, 8))
+
+    @property
+    def brick(self) -> tuple:
+        shape = (u32 * 3)()
+        _check(self.ctx.lib.garg_volume_splat_brick(self.h, shape))
+        return tuple(shape[:])
+
+    @brick.setter
+    def brick(self, shape):
+        _check(self.ctx.lib.garg_volume_set_splat_brick(self.h, *[int(v) for v in shape]))
 
     def splat(self, particles: Particles, exact: bool = True, frame: int = 0, stream=None):
         _check(self.ctx.lib.garg_splat_bin(self.h, particles.h, stream, 1 if exact else 0, frame))

@@ -1,11 +1,13 @@
-"""The batch autotuner: sweeps block size, resident waves per SIMD,
-compaction period and refill threshold of the persistent march on a
-workload, and persists the fastest configuration per (arch, workload) as
-<table>/<arch>/<workload>.json = {"config": {...}, "ms": ..., "date": ...},
-which callers load with `best()`. The table is GARG_AUTOTUNE_DIR, else
-gargantua_amd/autotune; `gargantua bench --sweep-batch` writes the same files.
+"""The autotuner. For the persistent march it sweeps block size, resident
+waves per SIMD, compaction period and refill threshold on a workload; for
+the exact (sorted, atomic-free) splat it sweeps the LDS brick shape of the
+gather (garg_volume_set_splat_brick). The fastest configuration per (arch,
+workload) persists as <table>/<arch>/<workload>.json = {"config": {...},
+"ms": ..., "date": ...}, which callers load with `best()` and
+`best_brick()`. The table is GARG_AUTOTUNE_DIR, else gargantua_amd/autotune;
+`gargantua bench --sweep-batch` writes the march's files in the same form.
 
-python -m gargantua_amd.autotune --workload headline|mixed
+python -m gargantua_amd.autotune --workload headline|mixed|splat_1M_128|splat_4M_256
 """
 from __future__ import annotations
 
@@ -104,14 +106,65 @@ def save(arch: str, workload: str, config: dict, ms: float):
         json.dump({"config": config, "ms": ms, "date": time.strftime("%Y-%m-%d")}, f, indent=1, sort_keys=True)
 
 
+# Exact-splat workloads: particles settled on the annulus, volume size.
+SPLAT_WORKLOADS = {"splat_1M_128": (1_000_000, 128), "splat_4M_256": (4_000_000, 256)}
+
+
+def sweep_bricks(ctx: api.Context, particles: api.Particles, volume: api.Volume, repeats: int = 5,
+                 log=print) -> list:
+    """Every brick shape of the exact deposit, timed over `repeats` deposits
+    (sort, stage and gather) with device events; fastest first."""
+    results = []
+    for shape in api.Volume.BRICKS:
+        volume.brick = shape
+        volume.splat(particles, exact=True)  # warm
+        a, b = ctx.event(), ctx.event()
+        a.record()
+        for _ in range(repeats):
+            volume.splat(particles, exact=True)
+        b.record()
+        b.sync()
+        ms = a.elapsed_ms(b) / repeats
+        results.append((list(shape), ms))
+        log(f"brick {shape[0]:2d}x{shape[1]:2d}x{shape[2]}: {ms:8.4f} ms")
+    volume.brick = api.Volume.BRICKS[0]
+    results.sort(key=lambda r: r[1])
+    return results
+
+
+def brick_for(arch: str, workload: str) -> tuple:
+    """The persisted brick shape for this arch and splat workload, else the
+    default (also for a shape the library does not build)."""
+    entry = load_entry(arch, workload)
+    shape = entry.get("config", {}).get("brick") if entry else None
+    if isinstance(shape, list) and tuple(shape) in api.Volume.BRICKS:
+        return tuple(shape)
+    return api.Volume.BRICKS[0]
+
+
+def best_brick(ctx: api.Context, workload: str) -> tuple:
+    return brick_for(ctx.info().arch.decode(), workload)
+
+
 def main():
-    ap = argparse.ArgumentParser(description="persistent march autotuner")
-    ap.add_argument("--workload", choices=["headline", "mixed"], default="headline")
+    ap = argparse.ArgumentParser(description="persistent march and exact splat autotuner")
+    ap.add_argument("--workload", choices=["headline", "mixed", *SPLAT_WORKLOADS], default="headline")
     ap.add_argument("--repeats", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="a reduced grid")
     ap.add_argument("--form", choices=sorted(FORMS), default="polar", help="integrator form of spinning requests")
     args = ap.parse_args()
     ctx = api.Context()
+    if args.workload in SPLAT_WORKLOADS:
+        count, size = SPLAT_WORKLOADS[args.workload]
+        p, v = workloads.disk(ctx, particles=count, volume=size)
+        results = sweep_bricks(ctx, p, v, max(args.repeats, 5))
+        shape, ms = results[0]
+        save(ctx.info().arch.decode(), args.workload, {"brick": shape}, ms)
+        print(json.dumps({"workload": args.workload, "best": {"brick": shape}, "ms": ms, "worst_ms": results[-1][1]}))
+        v.close()
+        p.close()
+        ctx.close()
+        return
     p, v = workloads.disk(ctx)
     reqs = [workloads.headline(ctx, v)] if args.workload == "headline" else workloads.mixed(ctx, v)
     grid = GRID

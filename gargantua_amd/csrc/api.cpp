@@ -575,7 +575,7 @@ int garg_volume_create(garg_ctx* ctx, uint32_t size, float half_extent, float ha
     v->voxels = (Voxel*)(b + sums);
     const uint64_t m = (uint64_t)size + 1u;
     const uint64_t cells = align256(m * m * m * 4u);
-    v->gather_bytes = 2u * cells + align256(splat_gather_bricks(size) * 4u);
+    v->gather_bytes = 2u * cells + align256(splat_gather_bricks_max(size) * 4u);
     rc = ctx_alloc(ctx, v->gather_bytes, &v->gather_base, "the deposit's cell tables");
     if (rc) {
         ctx_free(ctx, v->base, v->bytes);
@@ -586,6 +586,7 @@ int garg_volume_create(garg_ctx* ctx, uint32_t size, float half_extent, float ha
     v->gather.ends = (uint32_t*)((char*)v->gather_base + cells);
     v->gather.bricks = (uint32_t*)((char*)v->gather_base + 2u * cells);
     v->gather.stamp = 0;
+    for (int k = 0; k < 3; ++k) v->gather.brick[k] = kSplatBricks[0][k];
     hipError_t e = launch_fill_zero(v->base, v->bytes, ctx->stream);
     if (e == hipSuccess) e = launch_fill_zero(v->gather.bricks, v->gather_bytes - 2u * cells, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -645,6 +646,23 @@ int garg_splat_bin(garg_volume* v, garg_particles* p, garg_stream* stream, uint3
     } else {
         HIP_TRY(launch_splat_bin(a, s), "launching splatParticles");
     }
+    return GARG_OK;
+}
+
+int garg_volume_set_splat_brick(garg_volume* v, uint32_t x, uint32_t y, uint32_t z) {
+    REQUIRE(v, "volume is required.");
+    const uint32_t shape[3] = {x, y, z};
+    REQUIRE(splat_brick_supported(shape),
+            "brick %ux%ux%u is not one of the exact deposit's shapes (16x8x4, 8x8x4, 8x8x8, 16x16x4, 32x8x4, "
+            "16x8x8, 32x16x4, 16x16x8).",
+            x, y, z);
+    for (int k = 0; k < 3; ++k) v->gather.brick[k] = shape[k];
+    return GARG_OK;
+}
+
+int garg_volume_splat_brick(garg_volume* v, uint32_t shape[3]) {
+    REQUIRE(v && shape, "volume and shape are required.");
+    for (int k = 0; k < 3; ++k) shape[k] = v->gather.brick[k];
     return GARG_OK;
 }
 

@@ -155,9 +155,16 @@ struct SplatGather {
     uint32_t* ends;
     uint32_t* bricks;
     uint32_t stamp;
+    uint32_t brick[3];  // voxels per brick on x, y, z: one of kSplatBricks
 };
 size_t splat_sort_temp_bytes(uint32_t count);
-uint64_t splat_gather_bricks(uint32_t size);
+// The brick shapes the exact deposit is built for; the first is the default.
+constexpr int kSplatBrickShapes = 8;
+extern const uint32_t kSplatBricks[kSplatBrickShapes][3];
+bool splat_brick_supported(const uint32_t shape[3]);
+uint64_t splat_gather_bricks(uint32_t size, const uint32_t shape[3]);
+// Brick stamps a volume of this size needs for its finest supported shape.
+uint64_t splat_gather_bricks_max(uint32_t size);
 hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, const SplatGather& g,
                                hipStream_t stream);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);

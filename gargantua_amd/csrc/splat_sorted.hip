@@ -19,6 +19,8 @@
 // cell c deposits on each axis to max(c - 1, 0) (weight 1 - fraction) and
 // min(c, size - 1) (weight fraction), so voxel v draws from cells v and
 // v + 1 only.
+#include <algorithm>
+
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "device_common.hpp"
@@ -28,9 +30,10 @@ namespace garg {
 
 namespace {
 
-// Voxel bricks of the deposit, one workgroup each, and the consecutive
-// sorted particles one thread deposits.
-constexpr uint32_t BRICK_X = 16, BRICK_Y = 8, BRICK_Z = 4;
+// Threads of a brick's workgroup, and the consecutive sorted particles one
+// thread deposits. The brick's shape (voxels per axis, one workgroup per
+// brick, its sums held in LDS: 24 bytes per voxel) is a launch choice from
+// kSplatBricks; every shape gives the same bits.
 constexpr uint32_t BRICK_THREADS = 512;
 constexpr uint32_t CHUNK = 8;
 
@@ -113,8 +116,8 @@ GARG_DEV float corner_weight(float fx, float fy, float fz, uint32_t k) {
     return w * wz;
 }
 
-GARG_DEV uint32_t brick_of(uint32_t x, uint32_t y, uint32_t z, uint32_t nbx, uint32_t nby) {
-    return ((z / BRICK_Z) * nby + y / BRICK_Y) * nbx + x / BRICK_X;
+GARG_DEV uint32_t brick_of(uint32_t x, uint32_t y, uint32_t z, const uint32_t shape[3], uint32_t nbx, uint32_t nby) {
+    return ((z / shape[2]) * nby + y / shape[1]) * nbx + x / shape[0];
 }
 
 }  // namespace
@@ -156,14 +159,14 @@ __global__ __launch_bounds__(256) void splat_stage_kernel(SplatArgs a, SplatSort
     if (s > 0 && b.keys_out[s - 1u] == key) return;
     g.cells[key] = s;
     const int cx = (int)(key % m), cy = (int)((key / m) % m), cz = (int)(key / (m * m));
-    const uint32_t nbx = (n + BRICK_X - 1) / BRICK_X, nby = (n + BRICK_Y - 1) / BRICK_Y;
+    const uint32_t nbx = (n + g.brick[0] - 1) / g.brick[0], nby = (n + g.brick[1] - 1) / g.brick[1];
     // The cell's corners (equal on a clamped axis: the repeated store is
     // harmless).
     for (uint32_t k = 0; k < CIC_CORNERS; ++k) {
         uint32_t x = (uint32_t)((k & 1u) ? min(cx, (int)n - 1) : max(cx - 1, 0));
         uint32_t y = (uint32_t)((k & 2u) ? min(cy, (int)n - 1) : max(cy - 1, 0));
         uint32_t z = (uint32_t)((k & 4u) ? min(cz, (int)n - 1) : max(cz - 1, 0));
-        g.bricks[brick_of(x, y, z, nbx, nby)] = g.stamp;
+        g.bricks[brick_of(x, y, z, g.brick, nbx, nby)] = g.stamp;
     }
 }
 
@@ -173,6 +176,7 @@ __global__ __launch_bounds__(256) void splat_stage_kernel(SplatArgs a, SplatSort
 // many contiguous runs of the sorted order. The workgroup deposits the
 // corners that land in the brick into an LDS box and then adds the box to
 // the brick's sums: it is the only writer of those voxels in the launch.
+template <uint32_t BRICK_X, uint32_t BRICK_Y, uint32_t BRICK_Z>
 __global__ __launch_bounds__(BRICK_THREADS) void splat_brick_kernel(SplatArgs a, SplatSortBuffers b, SplatGather g) {
     constexpr uint32_t VOXELS = BRICK_X * BRICK_Y * BRICK_Z;
     constexpr uint32_t ROWS = (BRICK_Y + 1) * (BRICK_Z + 1);
@@ -345,10 +349,36 @@ size_t splat_sort_temp_bytes(uint32_t count) {
     return bytes;
 }
 
-uint64_t splat_gather_bricks(uint32_t size) {
-    return (uint64_t)((size + BRICK_X - 1) / BRICK_X) * ((size + BRICK_Y - 1) / BRICK_Y) *
-           ((size + BRICK_Z - 1) / BRICK_Z);
+const uint32_t kSplatBricks[kSplatBrickShapes][3] = {{16, 8, 4}, {8, 8, 4}, {8, 8, 8},   {16, 16, 4},
+                                                     {32, 8, 4}, {16, 8, 8}, {32, 16, 4}, {16, 16, 8}};
+
+bool splat_brick_supported(const uint32_t shape[3]) {
+    for (const auto& b : kSplatBricks)
+        if (b[0] == shape[0] && b[1] == shape[1] && b[2] == shape[2]) return true;
+    return false;
 }
+
+uint64_t splat_gather_bricks(uint32_t size, const uint32_t shape[3]) {
+    return (uint64_t)((size + shape[0] - 1) / shape[0]) * ((size + shape[1] - 1) / shape[1]) *
+           ((size + shape[2] - 1) / shape[2]);
+}
+
+uint64_t splat_gather_bricks_max(uint32_t size) {
+    uint64_t most = 0;
+    for (const auto& b : kSplatBricks) most = std::max(most, splat_gather_bricks(size, b));
+    return most;
+}
+
+namespace {
+
+template <uint32_t X, uint32_t Y, uint32_t Z>
+hipError_t launch_bricks(const SplatArgs& a, const SplatSortBuffers& b, const SplatGather& g, hipStream_t stream) {
+    const uint32_t shape[3] = {X, Y, Z};
+    splat_brick_kernel<X, Y, Z><<<dim3((uint32_t)splat_gather_bricks(a.size, shape)), BRICK_THREADS, 0, stream>>>(a, b, g);
+    return hipGetLastError();
+}
+
+}  // namespace
 
 hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, const SplatGather& g,
                                hipStream_t stream) {
@@ -364,8 +394,16 @@ hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, co
     splat_stage_kernel<<<dim3(blocks), 256, 0, stream>>>(a, b, g);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    splat_brick_kernel<<<dim3((uint32_t)splat_gather_bricks(a.size)), BRICK_THREADS, 0, stream>>>(a, b, g);
-    return hipGetLastError();
+    const uint32_t x = g.brick[0], y = g.brick[1], z = g.brick[2];
+    if (x == 16 && y == 8 && z == 4) return launch_bricks<16, 8, 4>(a, b, g, stream);
+    if (x == 8 && y == 8 && z == 4) return launch_bricks<8, 8, 4>(a, b, g, stream);
+    if (x == 8 && y == 8 && z == 8) return launch_bricks<8, 8, 8>(a, b, g, stream);
+    if (x == 16 && y == 16 && z == 4) return launch_bricks<16, 16, 4>(a, b, g, stream);
+    if (x == 32 && y == 8 && z == 4) return launch_bricks<32, 8, 4>(a, b, g, stream);
+    if (x == 16 && y == 8 && z == 8) return launch_bricks<16, 8, 8>(a, b, g, stream);
+    if (x == 32 && y == 16 && z == 4) return launch_bricks<32, 16, 4>(a, b, g, stream);
+    if (x == 16 && y == 16 && z == 8) return launch_bricks<16, 16, 8>(a, b, g, stream);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace garg

@@ -469,6 +469,33 @@ void Scene::close() {
 }
 
 // Validate.advance and the splat of Still.render (Still.swift:31-39).
+// The exact splat's LDS brick the autotuner chose for this deposit, from
+// <autotune_dir>/<arch>/splat_<N>_<size>.json ({"config": {"brick": [x, y, z]}},
+// N in millions when whole); false when there is no such entry.
+bool tuned_brick(const char* arch, int particles, int volume, uint32_t shape[3]) {
+    const std::string count = particles % 1000000 == 0 ? std::to_string(particles / 1000000) + "M"
+                                                       : std::to_string(particles);
+    const std::string path = autotune_dir() + "/" + arch + "/splat_" + count + "_" + std::to_string(volume) + ".json";
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    std::string text;
+    char buf[512];
+    for (size_t n; (n = fread(buf, 1, sizeof buf, f)) > 0;) text.append(buf, n);
+    fclose(f);
+    size_t at = text.find("\"brick\"");
+    if (at == std::string::npos || (at = text.find('[', at)) == std::string::npos) return false;
+    const char* p = text.c_str() + at + 1;
+    for (int k = 0; k < 3; ++k) {
+        char* end = nullptr;
+        long v = strtol(p, &end, 10);
+        if (end == p || v <= 0 || v > 64) return false;
+        shape[k] = (uint32_t)v;
+        p = end;
+        while (*p == ' ' || *p == ',' || *p == '\n') ++p;
+    }
+    return true;
+}
+
 Scene make_scene(const Configuration& c, int settle_substeps, bool exact_splat) {
     Scene s;
     check(garg_ctx_create(0, 0, &s.ctx), "creating the GPU context");
@@ -483,6 +510,9 @@ Scene make_scene(const Configuration& c, int settle_substeps, bool exact_splat) 
     check(garg_particles_step(s.particles, nullptr, (uint32_t)(settle_substeps / (int)SIM_SUBSTEPS)), "settling");
     check(garg_volume_create(s.ctx, (uint32_t)c.volume, GARG_F(VOLUME_HALF_EXTENT), 0.0f, &s.volume),
           "allocating the volume");
+    uint32_t brick[3];
+    if (exact_splat && tuned_brick(info.arch, c.particles, c.volume, brick))
+        check(garg_volume_set_splat_brick(s.volume, brick[0], brick[1], brick[2]), "setting the tuned brick");
     check(garg_splat_bin(s.volume, s.particles, nullptr, exact_splat ? 1u : 0u, 0), "splatting");
     check(garg_splat_resolve(s.volume, nullptr, 1u), "resolving");
     check(garg_tables_create(s.ctx, nullptr, GARG_F(VOLUME_HALF_EXTENT), &s.tables), "building sweep tables");

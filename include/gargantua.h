@@ -232,6 +232,16 @@ int garg_volume_clear(garg_volume* volume, garg_stream* stream);
  * the direct atomic path, which needs neither. */
 #define GARG_FRAME_AUTO 0xffffffffu
 int garg_splat_bin(garg_volume* volume, garg_particles* particles, garg_stream* stream, uint32_t exact, uint32_t frame);
+/* No counterpart in the reference (its deposit is one atomic pass): the
+ * brick shape of the exact deposit's gather, for the autotuner. Each brick of
+ * x * y * z voxels is one 512-thread workgroup that sums its corners in LDS
+ * (24 bytes per voxel) and owns those voxels' sums, so the shape trades LDS
+ * per workgroup, the halo of cells read around a brick and the number of
+ * workgroups. Shapes: 16x8x4 (the default), 8x8x4, 8x8x8, 16x16x4, 32x8x4,
+ * 16x8x8, 32x16x4, 16x16x8; GARG_E_ARGUMENT for any other. Every shape gives
+ * the same sums, bit for bit (integers added modulo 2^32). */
+int garg_volume_set_splat_brick(garg_volume* volume, uint32_t x, uint32_t y, uint32_t z);
+int garg_volume_splat_brick(garg_volume* volume, uint32_t shape[3]);
 /* resolveVolume (splat.metal:93-120); clear_sums = 1 also zeroes the sums. */
 int garg_splat_resolve(garg_volume* volume, garg_stream* stream, uint32_t clear_sums);
 /* Synchronous reads: sums (2 u32, 3 i32, 1 u32 per voxel, z-major) and the

@@ -23,3 +23,16 @@ def test_autotune_table_round_trip(tmp_path, monkeypatch):
     autotune.save("gfx950:test", "headline", cfg, 3.5)
     assert autotune.load_entry("gfx950:test", "headline")["config"] == cfg
     assert autotune.load_entry("gfx950:test", "mixed") is None
+
+
+def test_autotune_brick_table_round_trip(tmp_path, monkeypatch):
+    """The exact splat's LDS brick shape in the same table: the saved shape
+    is read back, and a missing entry or a shape the library does not build
+    falls back to the default brick."""
+    from gargantua_amd import api, autotune
+    monkeypatch.setenv("GARG_AUTOTUNE_DIR", str(tmp_path))
+    assert autotune.brick_for("gfx950:test", "splat_1M_128") == api.Volume.BRICKS[0]
+    autotune.save("gfx950:test", "splat_1M_128", {"brick": [32, 8, 4]}, 0.21)
+    assert autotune.brick_for("gfx950:test", "splat_1M_128") == (32, 8, 4)
+    autotune.save("gfx950:test", "splat_4M_256", {"brick": [4, 4, 4]}, 0.5)
+    assert autotune.brick_for("gfx950:test", "splat_4M_256") == api.Volume.BRICKS[0]

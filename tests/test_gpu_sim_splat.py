@@ -268,3 +268,40 @@ def test_sorted_exact_deposit_equals_direct_deposit(ctx):
     for o in (g, g2, s, eager, graphed, p, q):
         o.close()
 
+
+
+@pytest.mark.parametrize("size", [128, 96])
+def test_every_lds_brick_shape_gives_the_same_sums(ctx, size):
+    """The exact deposit's brick shape (the autotuner's LDS dimension) moves
+    work between workgroups and their LDS boxes but not the integer sums:
+    every shape deposits the direct (atomic, graph-captured) path's bits,
+    also on a size that is not a multiple of the larger bricks, and after
+    another shape's deposit into the same volume, whose brick stamps the
+    new shape must ignore."""
+    p = settled(ctx, 300_000, 7, 40)
+    ref = api.Volume(ctx, size, 30.0, 3.0)
+    s = ctx.stream()
+    ctx.sync()
+    ctx.graph_begin(s)
+    try:
+        ref.splat(p, exact=True, stream=s)
+    finally:
+        g = ctx.graph_end(s)
+    g.launch(s)
+    s.sync()
+    want = ref.read_sums()
+    assert want[2].sum() > 0
+    v = api.Volume(ctx, size, 30.0, 3.0)
+    assert v.brick == api.Volume.BRICKS[0]
+    for shape in api.Volume.BRICKS:
+        v.brick = shape
+        assert v.brick == shape
+        v.clear()
+        v.splat(p, exact=True)
+        ctx.sync()
+        got = v.read_sums()
+        assert all(np.array_equal(a, b) for a, b in zip(got, want)), shape
+    with pytest.raises(api.GargError, match="not one of the exact deposit's shapes"):
+        v.brick = (4, 4, 4)
+    for o in (g, s, ref, v, p):
+        o.close()
