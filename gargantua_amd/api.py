@@ -68,6 +68,11 @@ class RayProbeResult(C.Structure):
                 ("energy", f32), ("angular_momentum", f32)]
 
 
+class RayProbeFp64Result(C.Structure):
+    _fields_ = [("position", C.c_double * 3), ("direction", C.c_double * 3), ("drift", C.c_double),
+                ("steps", u64), ("outcome", u32), ("reserved", u32)]
+
+
 class MarchCounters(C.Structure):
     _fields_ = [("rays", u32), ("over_drift_budget", u32), ("exhausted", u32), ("captured", u32), ("steps", u64)]
 
@@ -135,6 +140,7 @@ SIGNATURES = {
     "garg_march_image": (C.c_int, [vp, vp, P(MarchUniforms), vp, vp, vp, u32, u32, u32]),
     "garg_probe": (C.c_int, [vp, vp, P(RayProbe), u32, P(MarchUniforms), vp, u32, P(RayProbeResult)]),
     "garg_probe_noalloc": (C.c_int, [vp, vp, P(RayProbe), u32, P(MarchUniforms), vp, u32, P(RayProbeResult)]),
+    "garg_probe_fp64": (C.c_int, [vp, vp, P(RayProbe), u32, P(GeodesicSettings), P(RayProbeFp64Result)]),
     "garg_request_rays": (u32, [u32, u32]),
     "garg_batch_config_default": (C.c_int, [vp, P(BatchConfig)]),
     "garg_march_batch": (C.c_int, [vp, vp, P(MarchRequest), u32, P(BatchConfig), P(MarchCounters)]),
@@ -297,6 +303,18 @@ class Context:
         h = vp()
         _check(self.lib.garg_graph_end(self.h, stream, C.byref(h)))
         return Graph(self, h)
+
+    def probe_fp64(self, probes, settings: GeodesicSettings):
+        """The reference's FP64 Schwarzschild integrator on the GPU
+        (garg_probe_fp64) for the float launches of `probes`."""
+        n = len(probes)
+        arr = (RayProbe * n)()
+        for i, (o, d) in enumerate(probes):
+            arr[i].origin[:] = [float(x) for x in o]
+            arr[i].direction[:] = [float(x) for x in d]
+        out = (RayProbeFp64Result * n)()
+        _check(self.lib.garg_probe_fp64(self.h, None, arr, n, C.byref(settings), out))
+        return out
 
     def probe(self, probes, u: MarchUniforms, tables=None, integrator: int = INTEGRATOR_FAST, noalloc: bool = False):
         """probeRays; noalloc=True goes through garg_probe_noalloc (at most

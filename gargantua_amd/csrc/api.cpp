@@ -1262,6 +1262,46 @@ int garg_deflection_table(garg_ctx* ctx, garg_stream* stream, double r_obs, uint
     return GARG_OK;
 }
 
+int garg_probe_fp64(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
+                    const garg_geodesic_settings* settings, garg_ray_probe_fp64_result* results) {
+    REQUIRE(ctx && settings && (count == 0 || (probes && results)), "ctx, settings, probes and results are required.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_probe_fp64");
+    REQUIRE(count <= 65536u, "at most 65536 FP64 probes per call.");
+    const garg_geodesic_settings& g = *settings;
+    REQUIRE(g.step_min > 0.0f && g.step_max >= g.step_min && g.empty_step_max >= g.step_min && g.step_factor > 0.0f &&
+                g.empty_step_factor > 0.0f,
+            "the step policy needs positive factors and step_min <= step_max.");
+    REQUIRE(g.capture_radius > R_SCHWARZSCHILD && g.escape_radius > g.capture_radius,
+            "the capture radius must lie outside r = 2 and inside the escape radius.");
+    REQUIRE(g.step_cap > 0u && g.step_cap <= (1u << 24), "the step cap must be in [1, 2^24].");
+    for (uint32_t i = 0; i < count; ++i) {
+        const float* o = probes[i].origin;
+        const double r = std::sqrt((double)o[0] * o[0] + (double)o[1] * o[1] + (double)o[2] * o[2]);
+        const float* d = probes[i].direction;
+        REQUIRE(r > g.capture_radius && std::isfinite(r), "probe %u starts at r = %g, inside the capture radius.", i, r);
+        REQUIRE(d[0] != 0.0f || d[1] != 0.0f || d[2] != 0.0f, "probe %u has no direction.", i);
+    }
+    if (count == 0) return GARG_OK;
+    const ProbeSettings64 g64{g.step_factor, g.step_min, g.step_max, g.capture_radius, g.escape_radius, g.step_cap,
+                              g.empty_step_factor, g.empty_step_max};
+    const uint64_t in_bytes = align256((uint64_t)count * sizeof(garg_ray_probe));
+    const uint64_t bytes = in_bytes + (uint64_t)count * sizeof(garg_ray_probe_fp64_result);
+    void* buf = nullptr;
+    int rc = ctx_alloc(ctx, bytes, &buf, "the FP64 probes");
+    if (rc) return rc;
+    garg_ray_probe* d_in = (garg_ray_probe*)buf;
+    garg_ray_probe_fp64_result* d_out = (garg_ray_probe_fp64_result*)((char*)buf + in_bytes);
+    hipStream_t s = stream_of(ctx, stream);
+    hipError_t e = hipMemcpyAsync(d_in, probes, (size_t)count * sizeof(garg_ray_probe), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_probe_fp64(d_in, count, g64, d_out, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(results, d_out, (size_t)count * sizeof(garg_ray_probe_fp64_result), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    ctx_free(ctx, buf, bytes);
+    HIP_TRY(e, "integrating the FP64 probes");
+    return GARG_OK;
+}
+
 int garg_shadow_boundary(garg_ctx* ctx, garg_stream* stream, double spin, double r_obs, double inclination_deg,
                          uint32_t azimuths, uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap,
                          double* edge_out, double* xi_out) {

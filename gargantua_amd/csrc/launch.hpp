@@ -212,6 +212,15 @@ struct WalkArgs {
     uint32_t capacity;
 };
 
+// The FP64 probe's policy: garg_geodesic_settings widened to double.
+struct ProbeSettings64 {
+    double step_factor, step_min, step_max, capture_radius, escape_radius;
+    uint64_t step_cap;
+    double empty_step_factor, empty_step_max;
+};
+hipError_t launch_probe_fp64(const garg_ray_probe* probes, uint32_t count, const ProbeSettings64& g,
+                             garg_ray_probe_fp64_result* out, hipStream_t stream);
+
 hipError_t launch_bake(const BakeArgs& a, hipStream_t stream);
 hipError_t launch_walk(const WalkArgs& a, hipStream_t stream);
 

@@ -1,6 +1,8 @@
 // FP64 kernels for gfx950: the Schwarzschild sweep tables the march jumps
 // with (Deflection.swift, DeflectionTables.swift), a log-spaced far-field
-// deflection table, and the Kerr shadow boundary by parallel multi-section.
+// deflection table, the Kerr shadow boundary by parallel multi-section, and
+// Schwarzschild probes integrated in double precision (the reference's FP64
+// integrator, which its validation judges the GPU's probes against).
 // Double precision stays on these because their stability needs it: the
 // sweep integrand is singular at the turning point and the shadow edge is a
 // separatrix where single precision rays change fate.
@@ -257,6 +259,136 @@ __global__ __launch_bounds__(64) void shadow_kernel(double* edge, double* xi, ui
             xi[az] = k.L / k.E;
         }
     }
+}
+
+// ---- FP64 Schwarzschild probes ---------------------------------------------
+// Oracle/Schwarzschild.swift's integrator, one ray per lane: the ray's plane
+// spanned by the launch radius e1 and the in-plane normal e2, the state
+// (v, r, phi) and its affine rates in ingoing Eddington-Finkelstein form,
+// classical RK4 (:121-136), the step length with the empty-leg coarse step
+// when the ray is farther from the disk slab than that step (:174-184), and
+// the energy / angular momentum drift against the launch (:200-220).
+
+struct PlaneRay64 {
+    double v, r, phi, vdot, rdot, phidot;
+};
+
+GARG_DEV PlaneRay64 plane_rates(const PlaneRay64& s) {
+    const double r = s.r, f = 1.0 - R_SCHWARZSCHILD / r;
+    const double w2 = s.phidot * s.phidot, vv_r2 = s.vdot * s.vdot / (r * r);
+    return {s.vdot,
+            s.rdot,
+            s.phidot,
+            r * w2 - vv_r2,
+            2.0 * s.rdot * s.vdot / (r * r) + f * (r * w2 - vv_r2),
+            -2.0 * s.rdot * s.phidot / r};
+}
+
+GARG_DEV PlaneRay64 plane_shift(const PlaneRay64& s, const PlaneRay64& k, double h) {
+    return {s.v + k.v * h, s.r + k.r * h, s.phi + k.phi * h, s.vdot + k.vdot * h, s.rdot + k.rdot * h,
+            s.phidot + k.phidot * h};
+}
+
+GARG_DEV void plane_rk4(PlaneRay64& s, double h) {
+    const PlaneRay64 k1 = plane_rates(s);
+    const PlaneRay64 k2 = plane_rates(plane_shift(s, k1, 0.5 * h));
+    const PlaneRay64 k3 = plane_rates(plane_shift(s, k2, 0.5 * h));
+    const PlaneRay64 k4 = plane_rates(plane_shift(s, k3, h));
+    const double w = h / 6.0;
+    s.v += (k1.v + 2.0 * k2.v + 2.0 * k3.v + k4.v) * w;
+    s.r += (k1.r + 2.0 * k2.r + 2.0 * k3.r + k4.r) * w;
+    s.phi += (k1.phi + 2.0 * k2.phi + 2.0 * k3.phi + k4.phi) * w;
+    s.vdot += (k1.vdot + 2.0 * k2.vdot + 2.0 * k3.vdot + k4.vdot) * w;
+    s.rdot += (k1.rdot + 2.0 * k2.rdot + 2.0 * k3.rdot + k4.rdot) * w;
+    s.phidot += (k1.phidot + 2.0 * k2.phidot + 2.0 * k3.phidot + k4.phidot) * w;
+}
+
+GARG_DEV double plane_energy(const PlaneRay64& s) { return (1.0 - R_SCHWARZSCHILD / s.r) * s.vdot - s.rdot; }
+GARG_DEV double plane_angular_momentum(const PlaneRay64& s) { return s.r * s.r * s.phidot; }
+
+__global__ __launch_bounds__(64) void probe_fp64_kernel(const garg_ray_probe* probes, uint32_t count,
+                                                        ProbeSettings64 g, garg_ray_probe_fp64_result* out) {
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    if (i >= count) return;
+    double o[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+        o[k] = probes[i].origin[k];
+        d[k] = probes[i].direction[k];
+    }
+    // The static observer's launch (Schwarzschild.swift:76-97): unit local
+    // energy, the direction split into radial and in-plane tangential parts.
+    const double r0 = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+    const double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double e1[3], e2[3], unit[3];
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = o[k] / r0;
+        unit[k] = d[k] / dl;
+    }
+    const double radial = unit[0] * e1[0] + unit[1] * e1[1] + unit[2] * e1[2];
+    for (int k = 0; k < 3; ++k) e2[k] = unit[k] - radial * e1[k];
+    double across = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+    if (across < 1e-12) {
+        // A purely radial ray: any plane through it.
+        double helper[3] = {fabs(e1[0]) < 0.9 ? 1.0 : 0.0, fabs(e1[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+        const double along = helper[0] * e1[0] + helper[1] * e1[1] + helper[2] * e1[2];
+        for (int k = 0; k < 3; ++k) e2[k] = helper[k] - along * e1[k];
+        const double n = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+        for (int k = 0; k < 3; ++k) e2[k] /= n;
+        across = 0.0;
+    } else {
+        for (int k = 0; k < 3; ++k) e2[k] /= across;
+    }
+    const double f0 = 1.0 - R_SCHWARZSCHILD / r0;
+    PlaneRay64 s{0.0, r0, 0.0, (1.0 + radial) / sqrt(f0), sqrt(f0) * radial, across / r0};
+    const double energy = plane_energy(s), momentum = plane_angular_momentum(s);
+    const double momentum_scale = fmax(fabs(momentum), 1e-3);
+    double drift = 0.0;
+    uint64_t steps = 0;
+    uint32_t outcome = GARG_RAY_EXHAUSTED;
+    while (steps < g.step_cap) {
+        double sn, cs;
+        sincos(s.phi, &sn, &cs);
+        const double z = s.r * (cs * e1[2] + sn * e2[2]);
+        const double coarse = fmin(fmax(g.empty_step_factor * s.r, g.step_min), g.empty_step_max);
+        const double h = fabs(z) > DISK_SLAB_HALF_HEIGHT + coarse ? coarse
+                                                                  : fmin(fmax(g.step_factor * s.r, g.step_min), g.step_max);
+        plane_rk4(s, h);
+        ++steps;
+        drift = fmax(drift, fmax(fabs(plane_energy(s) - energy) / fabs(energy),
+                                 fabs(plane_angular_momentum(s) - momentum) / momentum_scale));
+        if (s.r <= g.capture_radius) {
+            outcome = GARG_RAY_CAPTURED;
+            break;
+        }
+        if (s.r >= g.escape_radius) {
+            outcome = GARG_RAY_ESCAPED;
+            break;
+        }
+    }
+    // Position in the plane, and the direction a static observer there sees
+    // (Schwarzschild.swift:58-67).
+    double sn, cs;
+    sincos(s.phi, &sn, &cs);
+    const double f = fmax(1.0 - R_SCHWARZSCHILD / s.r, 1e-6);
+    double dir[3], len2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double rad = cs * e1[k] + sn * e2[k], tangent = -sn * e1[k] + cs * e2[k];
+        out[i].position[k] = s.r * rad;
+        dir[k] = s.rdot / sqrt(f) * rad + s.r * s.phidot * tangent;
+        len2 += dir[k] * dir[k];
+    }
+    const double len = sqrt(len2);
+    for (int k = 0; k < 3; ++k) out[i].direction[k] = len > 0.0 ? dir[k] / len : cs * e1[k] + sn * e2[k];
+    out[i].drift = drift;
+    out[i].steps = steps;
+    out[i].outcome = outcome;
+    out[i].reserved = 0u;
+}
+
+hipError_t launch_probe_fp64(const garg_ray_probe* probes, uint32_t count, const ProbeSettings64& g,
+                             garg_ray_probe_fp64_result* out, hipStream_t stream) {
+    probe_fp64_kernel<<<dim3((count + 63u) / 64u), 64, 0, stream>>>(probes, count, g, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_sweep_table(float* out, uint32_t n, double radius, double low, double high, uint32_t intervals,

@@ -383,6 +383,25 @@ int garg_shadow_boundary(garg_ctx* ctx, garg_stream* stream, double spin, double
                          uint32_t azimuths, uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap,
                          double* edge_out, double* xi_out);
 
+/* The reference's FP64 Schwarzschild integrator (Oracle/Schwarzschild.swift:
+ * 76-97 launch, 121-136 RK4, 174-184 step length with the empty-leg coarse
+ * step, 200-220 the loop and its energy / angular momentum drift) on the GPU
+ * in double precision, one ray per lane, for the probes' launches and the
+ * settings widened to double. The reference judges its parity and sky
+ * goldens against that integrator on the CPU (Validate.swift:179-200,
+ * 293-320); `gargantua validate` judges them against this. Synchronous; at
+ * most 65536 probes. */
+typedef struct {
+    double position[3];
+    double direction[3]; /* static-observer direction at the end point */
+    double drift;        /* max relative E / L drift over the path */
+    uint64_t steps;
+    uint32_t outcome;    /* GARG_RAY_CAPTURED / ESCAPED / EXHAUSTED */
+    uint32_t reserved;
+} garg_ray_probe_fp64_result;
+int garg_probe_fp64(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
+                    const garg_geodesic_settings* settings, garg_ray_probe_fp64_result* results);
+
 /* ---- measurement ------------------------------------------------------------ */
 
 /* No counterpart in the reference: the ceiling the splat bin is measured

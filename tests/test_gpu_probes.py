@@ -193,3 +193,56 @@ def test_shadow_boundary_fp64_matches_oracle_and_bardeen(ctx):
     assert np.all(edge > 0) and np.all(np.isfinite(edge))
     # The shadow is symmetric about the equatorial plane at 90 degrees.
     assert np.max(np.abs(edge[1:180] - edge[359:180:-1])) < 1e-6
+
+
+def sky_settings():
+    """The sky golden's far integration (Validate.swift:293-305): step
+    factor 0.01 up to 20, escape at oracleRadius, 400,000 steps, the coarse
+    empty-leg factor left at the default step factor."""
+    g = api.settings("still", step_cap=400_000, escape_radius=load_golden("sky")["parameters"]["oracleRadius"])
+    g.step_factor, g.step_max, g.empty_step_max = 0.01, 20.0, 20.0
+    g.empty_step_factor = 0.02
+    return g
+
+
+@pytest.mark.parametrize("kind", ["parity", "sky"])
+def test_fp64_probes_match_the_cpu_oracle(ctx, kind):
+    """garg_probe_fp64, the reference's FP64 Schwarzschild integrator on the
+    GPU that `gargantua validate` judges the parity and sky goldens against,
+    against the CPU oracle's restatement of the same integrator with the
+    same (float-valued) settings, on those goldens' 8 x 8 camera grids:
+    equal outcomes and step counts, end points and end directions within
+    1e-9 (the two differ only in the rounding of sqrt, sin, cos and the
+    rate expressions), and drifts within 1e-9 of each other."""
+    g = api.settings("still") if kind == "parity" else sky_settings()
+    p = load_golden(kind)["parameters"]
+    launches = scenes.camera_grid(api.orbit_camera(), int(p["columns"]), int(p["rows"]), 960 / 540)
+    res = ctx.probe_fp64(launches, g)
+    s = O.settings("still")
+    for name in ("step_factor", "step_min", "step_max", "capture_radius", "escape_radius", "empty_step_factor",
+                 "empty_step_max"):
+        setattr(s, name, float(getattr(g, name)))
+    s.step_cap = int(g.step_cap)
+    worst_p = worst_d = worst_drift = 0.0
+    escaped = 0
+    for (o, d), r in zip(launches, res):
+        o32 = np.array(o, np.float32).astype(np.float64)
+        d32 = np.array(d, np.float32).astype(np.float64)
+        ref = O.schw_integrate(O.schw_launch(o32, d32), s)
+        assert r.outcome == ref.outcome and r.steps == ref.steps, (r.outcome, ref.outcome, r.steps, ref.steps)
+        pos = O.schw_position(ref.ray)
+        worst_p = max(worst_p, np.linalg.norm(np.array(r.position[:]) - pos) / max(np.linalg.norm(pos), 1.0))
+        worst_d = max(worst_d, np.linalg.norm(np.array(r.direction[:]) - O.schw_direction(ref.ray)))
+        worst_drift = max(worst_drift, abs(r.drift - ref.drift))
+        escaped += r.outcome == api.RAY_ESCAPED
+    print(f"{kind}: FP64 GPU vs CPU oracle: end point {worst_p:.2e}, direction {worst_d:.2e}, drift {worst_drift:.2e}, "
+          f"{escaped} of {len(launches)} escaped")
+    assert escaped > 10 and worst_p < 1e-9 and worst_d < 1e-9 and worst_drift < 1e-9
+
+
+def test_fp64_probe_refuses_bad_launches(ctx):
+    g = api.settings("still")
+    with pytest.raises(api.GargError, match="inside the capture radius"):
+        ctx.probe_fp64([((1.0, 0.0, 0.0), (0.0, 1.0, 0.0))], g)
+    with pytest.raises(api.GargError, match="has no direction"):
+        ctx.probe_fp64([((50.0, 0.0, 0.0), (0.0, 0.0, 0.0))], g)
