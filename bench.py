@@ -38,10 +38,10 @@ METRIC = "RK4 geodesic steps/s at 1080p a=0.9 vs plain march; p50/p99 render lat
 FLOPS_PER_KERR_STEP = 509
 FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak (packed)
 FP32_PEAK_NONPACKED_TFLOPS = 78.6
-# HBM bytes per launch of the batched Kerr march at this frame, from two
-# rocprofv3 counter runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a
-# 256 MiB device copy reads); profiles/march_kerr_1080p_a0.9_counters.md.
-TRAFFIC_BYTES_PER_LAUNCH = 479.0e6
+# HBM bytes per launch of the batched polar-form Kerr march at this frame,
+# from two rocprofv3 counter runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the
+# factors a 256 MiB device copy reads); profiles/march_kerr_1080p_a0.9_counters.md.
+TRAFFIC_BYTES_PER_LAUNCH = 416.3e6
 
 
 def parse():
@@ -132,7 +132,7 @@ def cpu_baseline(u, vol, every: int):
     _, total = scenes.oracle_pixels(p, xy, field, threads)
     dt = time.perf_counter() - t0
     return {"value": total / dt, "unit": "geodesic steps/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle Kerr march with gas on every {every}th row of the 1920x1080 frame "
+            "sample": f"FP64 oracle Kerr march with gas on {'every row' if every == 1 else f'every {every}th row'} of the 1920x1080 frame "
                       f"({len(xy)} rays, {total} steps, {dt:.1f} s)",
             "cpu": _cpu_model(), "nproc": os.cpu_count(), "cores_allowed": allowed,
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
@@ -602,7 +602,12 @@ def pass_leg(ctx, api, particles, vol):
         direct = ctx.graph_end()
     direct_ms = timed(lambda: direct.launch(), 10)
     direct.close()
-    resolve_ms = timed(lambda: (vol.splat(particles, exact=True), vol.resolve(clear=True)), 10) - bin_ms
+    # The resolve alone: its bytes do not depend on the sums' values, and a
+    # difference of two timings can fall below its 20 us. Then one deposit
+    # and resolve leave the volume as the other legs expect it.
+    resolve_ms = timed(lambda: vol.resolve(clear=True), 20)
+    vol.splat(particles, exact=True)
+    vol.resolve(clear=True)
     sim_bytes = 64 * n
     resolve_bytes = (24 + 16 + 24) * voxels
     # The ceiling for the bin: the same count of no-return atomics (8 corners
@@ -647,7 +652,7 @@ def max_preset_passes(ctx, api, timed):
         v.splat(p, exact=False, frame=frame[0])
 
     splat_ms = timed(stochastic, 20)
-    resolve_ms = timed(lambda: (stochastic(), v.resolve(clear=True)), 10) - splat_ms
+    resolve_ms = timed(lambda: v.resolve(clear=True), 20)
     ceiling = ctx.atomic_rate(6 * voxels, 6, n)
     out = {"sim": {"ms": sim_ms, "bytes": 64 * n, "GB_per_s": 64 * n / (sim_ms * 1e-3) / 1e9,
                    "hbm_frac": 64 * n / (sim_ms * 1e-3) / 8e12},

@@ -239,6 +239,15 @@ void run_serve(const Options& o) {
                 continue;
             }
             if (o.serve_max_wait > 0) {
+                // Held back by the HBM budget or the batch's slots for longer
+                // than the wait: shed, so the queue stays bounded too.
+                if (now - r.arrival > o.serve_max_wait) {
+                    r.done = now;
+                    r.shed = true;
+                    ++shed;
+                    pending.pop_front();
+                    continue;
+                }
                 double ahead = work_left(r);
                 for (int id : active) ahead += work_left(all[id]);
                 const double rate = march_seconds > 0.5 ? total_steps / march_seconds : 5e10;
@@ -284,13 +293,14 @@ void run_serve(const Options& o) {
         // The order the budget is handed out in: shortest remaining work
         // first (rays left over all frames, a still-budget ray counted as two
         // since it takes about twice the steps; ties in arrival order), or
-        // arrival order. With --max-wait, shortest first ages: a request's
-        // key is its predicted work left (steps) less what the GPU marches in
-        // the time it has been admitted, so a long sequence overtakes newer,
-        // shorter arrivals once it has waited about its own length, instead
-        // of waiting as long as they keep coming, while short requests still
-        // go first against anything that has not waited. Ray results do not
-        // depend on the order.
+        // arrival order. Ray results do not depend on it. Shortest first
+        // gives the lowest median and p99 on the serving stream; its cost is
+        // the tail: a long sequence gets what newer, shorter arrivals leave.
+        // (Serving requests that have waited longest first, or giving the
+        // oldest an eighth of each tick, or ageing the order, bounded that
+        // tail less well than it cost the p99: the admission estimate of a
+        // request's own work can be several times short for long sequences
+        // full of exhausted rays.)
         std::vector<int> order(active);
         if (o.serve_policy == "srpt") {
             auto work = [&](int id) {
@@ -298,13 +308,7 @@ void run_serve(const Options& o) {
                 const uint64_t left = (uint64_t)(r.frames - r.frame - 1) * r.rays + (r.rays - r.issued);
                 return r.budget > STEP_CAP_INTERACTIVE ? 2 * left : left;
             };
-            if (o.serve_max_wait > 0) {
-                const double rate = march_seconds > 0.5 ? total_steps / march_seconds : 5e10;
-                auto key = [&](int id) { return work_left(all[id]) - rate * (now - all[id].admitted); };
-                std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
-            } else {
-                std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) < work(y); });
-            }
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) < work(y); });
         }
         for (int id : order) {
             Request& r = all[id];
