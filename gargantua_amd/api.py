@@ -73,6 +73,11 @@ class RayProbeFp64Result(C.Structure):
                 ("steps", u64), ("outcome", u32), ("reserved", u32)]
 
 
+class RenderCameraFp64(C.Structure):
+    _fields_ = [("position", C.c_double * 3), ("right", C.c_double * 3), ("up", C.c_double * 3),
+                ("forward", C.c_double * 3), ("tan_half_fov", C.c_double * 2)]
+
+
 class MarchCounters(C.Structure):
     _fields_ = [("rays", u32), ("over_drift_budget", u32), ("exhausted", u32), ("captured", u32), ("steps", u64)]
 
@@ -141,6 +146,8 @@ SIGNATURES = {
     "garg_probe": (C.c_int, [vp, vp, P(RayProbe), u32, P(MarchUniforms), vp, u32, P(RayProbeResult)]),
     "garg_probe_noalloc": (C.c_int, [vp, vp, P(RayProbe), u32, P(MarchUniforms), vp, u32, P(RayProbeResult)]),
     "garg_probe_fp64": (C.c_int, [vp, vp, P(RayProbe), u32, P(GeodesicSettings), P(RayProbeFp64Result)]),
+    "garg_render_fp64": (C.c_int, [vp, vp, vp, P(RenderCameraFp64), u32, u32, P(GeodesicSettings), C.c_double, u32,
+                                   P(C.c_double)]),
     "garg_request_rays": (u32, [u32, u32]),
     "garg_batch_config_default": (C.c_int, [vp, P(BatchConfig)]),
     "garg_march_batch": (C.c_int, [vp, vp, P(MarchRequest), u32, P(BatchConfig), P(MarchCounters)]),
@@ -156,6 +163,7 @@ SIGNATURES = {
     "garg_shadow_boundary": (C.c_int, [vp, vp, f64, f64, f64, u32, u32, u32, u32, vp, vp]),
     "garg_particles_dispatches": (C.c_int, [vp, P(u32)]),
     "garg_atomic_rate": (C.c_int, [vp, vp, u32, u32, u32, P(f64)]),
+    "garg_valu_rate": (C.c_int, [vp, vp, u32, u32, P(f64)]),
     "garg_graph_begin": (C.c_int, [vp, vp]),
     "garg_graph_end": (C.c_int, [vp, vp, P(vp)]),
     "garg_graph_launch": (C.c_int, [vp, vp]),
@@ -295,6 +303,12 @@ class Context:
         _check(self.lib.garg_atomic_rate(self.h, None, words, adds_per_thread, threads, C.byref(r)))
         return r.value
 
+    def valu_rate(self, iterations: int = 2048, threads: int = 1 << 21) -> float:
+        """Non-packed FP32 fused multiply-add flops per second (garg_valu_rate)."""
+        r = f64()
+        _check(self.lib.garg_valu_rate(self.h, None, iterations, threads, C.byref(r)))
+        return r.value
+
     def graph_begin(self, stream=None):
         """Record every launch enqueued on `stream` until graph_end."""
         _check(self.lib.garg_graph_begin(self.h, stream))
@@ -314,6 +328,21 @@ class Context:
             arr[i].direction[:] = [float(x) for x in d]
         out = (RayProbeFp64Result * n)()
         _check(self.lib.garg_probe_fp64(self.h, None, arr, n, C.byref(settings), out))
+        return out
+
+    def render_fp64(self, volume, cam, width: int, height: int, settings: GeodesicSettings,
+                    opacity: float = 0.0005, redshift: bool = True) -> np.ndarray:
+        """The reference Oracle's lensed render of `volume` on the GPU in
+        double precision (garg_render_fp64): linear rgb, (height, width, 3).
+        `cam` is an orbit camera as orbit_camera() returns it."""
+        c = RenderCameraFp64()
+        for k in ("position", "right", "up", "forward"):
+            getattr(c, k)[:] = [float(x) for x in cam[k]]
+        th = math.tan(float(cam["fov_y"]) * 0.5)
+        c.tan_half_fov[:] = [th * width / height, th]
+        out = np.zeros((height, width, 3), np.float64)
+        _check(self.lib.garg_render_fp64(self.h, None, volume.h, C.byref(c), width, height, C.byref(settings),
+                                         opacity, 1 if redshift else 0, out.ctypes.data_as(P(C.c_double))))
         return out
 
     def probe(self, probes, u: MarchUniforms, tables=None, integrator: int = INTEGRATOR_FAST, noalloc: bool = False):

@@ -1302,6 +1302,58 @@ int garg_probe_fp64(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* pr
     return GARG_OK;
 }
 
+int garg_render_fp64(garg_ctx* ctx, garg_stream* stream, const garg_volume* volume,
+                     const garg_render_camera_fp64* camera, uint32_t width, uint32_t height,
+                     const garg_geodesic_settings* settings, double opacity, uint32_t redshift, double* rgb) {
+    REQUIRE(ctx && volume && camera && settings && rgb, "ctx, volume, camera, settings and rgb are required.");
+    REQUIRE(volume->ctx == ctx, "the volume belongs to another context.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_render_fp64");
+    REQUIRE(width > 0 && height > 0 && (uint64_t)width * height <= (1u << 22),
+            "the FP64 render takes 1 to 2^22 pixels (%u x %u asked).", width, height);
+    const garg_geodesic_settings& g = *settings;
+    REQUIRE(g.step_min > 0.0f && g.step_max >= g.step_min && g.empty_step_max >= g.step_min && g.step_factor > 0.0f &&
+                g.empty_step_factor > 0.0f,
+            "the step policy needs positive factors and step_min <= step_max.");
+    REQUIRE(g.capture_radius > R_SCHWARZSCHILD && g.escape_radius > g.capture_radius,
+            "the capture radius must lie outside r = 2 and inside the escape radius.");
+    REQUIRE(g.step_cap > 0u && g.step_cap <= (1u << 24), "the step cap must be in [1, 2^24].");
+    REQUIRE(opacity >= 0.0 && std::isfinite(opacity), "the opacity must be finite and non-negative.");
+    const double* p = camera->position;
+    const double r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    REQUIRE(r > g.capture_radius && std::isfinite(r), "the camera sits at r = %g, inside the capture radius.", r);
+    Render64 a{};
+    a.voxels = volume->voxels;
+    a.size = volume->size;
+    a.half_xy = volume->half_xy;
+    a.half_z = volume->half_z;
+    for (int k = 0; k < 3; ++k) {
+        a.position[k] = camera->position[k];
+        a.right[k] = camera->right[k];
+        a.up[k] = camera->up[k];
+        a.forward[k] = camera->forward[k];
+    }
+    a.tan_half_fov[0] = camera->tan_half_fov[0];
+    a.tan_half_fov[1] = camera->tan_half_fov[1];
+    a.width = width;
+    a.height = height;
+    a.opacity = opacity;
+    a.redshift = redshift ? 1u : 0u;
+    a.settings = ProbeSettings64{g.step_factor, g.step_min, g.step_max, g.capture_radius, g.escape_radius, g.step_cap,
+                                 g.empty_step_factor, g.empty_step_max};
+    const uint64_t bytes = (uint64_t)width * height * 3u * sizeof(double);
+    void* buf = nullptr;
+    int rc = ctx_alloc(ctx, bytes, &buf, "the FP64 render");
+    if (rc) return rc;
+    a.rgb = (double*)buf;
+    hipStream_t s = stream_of(ctx, stream);
+    hipError_t e = launch_render_fp64(a, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgb, buf, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    ctx_free(ctx, buf, bytes);
+    HIP_TRY(e, "rendering in FP64");
+    return GARG_OK;
+}
+
 int garg_shadow_boundary(garg_ctx* ctx, garg_stream* stream, double spin, double r_obs, double inclination_deg,
                          uint32_t azimuths, uint32_t rounds, uint32_t rays_per_round, uint32_t step_cap,
                          double* edge_out, double* xi_out) {
@@ -1410,6 +1462,34 @@ int garg_atomic_rate(garg_ctx* ctx, garg_stream* stream, uint32_t words, uint32_
     ctx_free(ctx, buffer, (uint64_t)words * 4u);
     HIP_TRY(e, "timing the atomic kernel");
     *adds_per_second = ms > 0.0f ? (double)threads * adds_per_thread / (ms * 1e-3) : 0.0;
+    return GARG_OK;
+}
+
+int garg_valu_rate(garg_ctx* ctx, garg_stream* stream, uint32_t iterations, uint32_t threads,
+                   double* flops_per_second) {
+    REQUIRE(ctx && flops_per_second, "ctx and flops_per_second are required.");
+    REQUIRE(iterations > 0 && threads > 0, "iterations and threads must be positive.");
+    REFUSE_WHILE_CAPTURING(ctx, stream, "garg_valu_rate");
+    hipStream_t s = stream_of(ctx, stream);
+    void* buffer = nullptr;
+    uint64_t bytes = align256((uint64_t)threads * 4u);
+    int rc = ctx_alloc(ctx, bytes, &buffer, "the FMA output");
+    if (rc) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = launch_valu_rate((float*)buffer, threads, iterations, s);  // warm
+    if (e == hipSuccess) e = hipEventRecord(a, s);
+    if (e == hipSuccess) e = launch_valu_rate((float*)buffer, threads, iterations, s);
+    if (e == hipSuccess) e = hipEventRecord(b, s);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    ctx_free(ctx, buffer, bytes);
+    HIP_TRY(e, "timing the FMA kernel");
+    *flops_per_second = ms > 0.0f ? 2.0 * 32.0 * threads * (double)iterations / (ms * 1e-3) : 0.0;
     return GARG_OK;
 }
 

@@ -170,6 +170,8 @@ hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, co
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
 hipError_t launch_fill_zero(void* ptr, uint64_t bytes, hipStream_t stream);
 hipError_t launch_atomic_rate(uint32_t* buffer, uint32_t words, uint32_t adds, uint32_t threads, hipStream_t stream);
+// `threads` x `iterations` x 32 independent FP32 fused multiply-adds.
+hipError_t launch_valu_rate(float* out, uint32_t threads, uint32_t iterations, hipStream_t stream);
 
 // FP64 sweep tables: table[i] for the sweep from `radius` over b at table
 // coordinate i / (n - 1) between low and high; kind 0 = sweep to infinity,
@@ -220,6 +222,22 @@ struct ProbeSettings64 {
 };
 hipError_t launch_probe_fp64(const garg_ray_probe* probes, uint32_t count, const ProbeSettings64& g,
                              garg_ray_probe_fp64_result* out, hipStream_t stream);
+
+// The FP64 lensed render of a resolved volume: one pixel per lane, linear
+// rgb (3 doubles per pixel, row major) into `rgb`.
+struct Render64 {
+    const Voxel* voxels;
+    uint32_t size;
+    double half_xy, half_z;
+    double position[3], right[3], up[3], forward[3];
+    double tan_half_fov[2];
+    uint32_t width, height;
+    double opacity;
+    uint32_t redshift;
+    ProbeSettings64 settings;
+    double* rgb;
+};
+hipError_t launch_render_fp64(const Render64& a, hipStream_t stream);
 
 hipError_t launch_bake(const BakeArgs& a, hipStream_t stream);
 hipError_t launch_walk(const WalkArgs& a, hipStream_t stream);

@@ -213,4 +213,34 @@ hipError_t launch_probe(const ProbeArgs& a, uint32_t integrator, hipStream_t str
     return hipGetLastError();
 }
 
+// The FP32 ceiling the march is measured against: every thread runs
+// `iterations` rounds of 32 independent fused multiply-adds (one non-packed
+// v_fma_f32 each: packed FP32 instructions are disabled in this build) and
+// stores their sum, so that none of them is dead code. Thirty-two chains
+// and a twice-unrolled loop leave the issue of the FMAs as the only bound.
+constexpr int VALU_RATE_CHAINS = 32;
+
+__global__ __launch_bounds__(256) void valu_rate_kernel(float* out, uint32_t threads, uint32_t iterations, float m,
+                                                       float c) {
+    uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= threads) return;
+    float x[VALU_RATE_CHAINS];
+#pragma unroll
+    for (int j = 0; j < VALU_RATE_CHAINS; ++j) x[j] = (float)(t & 1023u) * 1e-3f + (float)j;
+#pragma unroll 2
+    for (uint32_t i = 0; i < iterations; ++i) {
+#pragma unroll
+        for (int j = 0; j < VALU_RATE_CHAINS; ++j) x[j] = __builtin_fmaf(x[j], m, c);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < VALU_RATE_CHAINS; ++j) s += x[j];
+    out[t] = s;
+}
+
+hipError_t launch_valu_rate(float* out, uint32_t threads, uint32_t iterations, hipStream_t stream) {
+    valu_rate_kernel<<<dim3((threads + 255u) / 256u), 256, 0, stream>>>(out, threads, iterations, 0.99999994f, 1e-7f);
+    return hipGetLastError();
+}
+
 }  // namespace garg

@@ -306,20 +306,13 @@ GARG_DEV void plane_rk4(PlaneRay64& s, double h) {
 GARG_DEV double plane_energy(const PlaneRay64& s) { return (1.0 - R_SCHWARZSCHILD / s.r) * s.vdot - s.rdot; }
 GARG_DEV double plane_angular_momentum(const PlaneRay64& s) { return s.r * s.r * s.phidot; }
 
-__global__ __launch_bounds__(64) void probe_fp64_kernel(const garg_ray_probe* probes, uint32_t count,
-                                                        ProbeSettings64 g, garg_ray_probe_fp64_result* out) {
-    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
-    if (i >= count) return;
-    double o[3], d[3];
-    for (int k = 0; k < 3; ++k) {
-        o[k] = probes[i].origin[k];
-        d[k] = probes[i].direction[k];
-    }
-    // The static observer's launch (Schwarzschild.swift:76-97): unit local
-    // energy, the direction split into radial and in-plane tangential parts.
+// The static observer's launch (Schwarzschild.swift:76-97): unit local
+// energy, the direction split into radial and in-plane tangential parts;
+// e1 and e2 span the ray's plane.
+GARG_DEV PlaneRay64 plane_launch(const double o[3], const double d[3], double e1[3], double e2[3]) {
     const double r0 = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
     const double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    double e1[3], e2[3], unit[3];
+    double unit[3];
     for (int k = 0; k < 3; ++k) {
         e1[k] = o[k] / r0;
         unit[k] = d[k] / dl;
@@ -339,7 +332,20 @@ __global__ __launch_bounds__(64) void probe_fp64_kernel(const garg_ray_probe* pr
         for (int k = 0; k < 3; ++k) e2[k] /= across;
     }
     const double f0 = 1.0 - R_SCHWARZSCHILD / r0;
-    PlaneRay64 s{0.0, r0, 0.0, (1.0 + radial) / sqrt(f0), sqrt(f0) * radial, across / r0};
+    return PlaneRay64{0.0, r0, 0.0, (1.0 + radial) / sqrt(f0), sqrt(f0) * radial, across / r0};
+}
+
+__global__ __launch_bounds__(64) void probe_fp64_kernel(const garg_ray_probe* probes, uint32_t count,
+                                                        ProbeSettings64 g, garg_ray_probe_fp64_result* out) {
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    if (i >= count) return;
+    double o[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+        o[k] = probes[i].origin[k];
+        d[k] = probes[i].direction[k];
+    }
+    double e1[3], e2[3];
+    PlaneRay64 s = plane_launch(o, d, e1, e2);
     const double energy = plane_energy(s), momentum = plane_angular_momentum(s);
     const double momentum_scale = fmax(fabs(momentum), 1e-3);
     double drift = 0.0;
@@ -383,6 +389,143 @@ __global__ __launch_bounds__(64) void probe_fp64_kernel(const garg_ray_probe* pr
     out[i].steps = steps;
     out[i].outcome = outcome;
     out[i].reserved = 0u;
+}
+
+// ---- FP64 lensed render ----------------------------------------------------
+// Oracle/LensedRender.swift:46-91 on the GPU, one pixel per lane: the camera
+// ray through the pixel centre, the FP64 integrator above with the step
+// length taken at the previous point's height, and the gas gathered in
+// double precision from the resolved volume with VolumeField.sample's
+// trilinear weights on texel centres and zero outside (VolumeField.swift:
+// 40-63): the Doppler factor cubed blended with the gravitational one by
+// density, Beer-Lambert opacity, front to back until the transmittance
+// floor. The reference's beaming golden judges the GPU march's image against
+// this render (Validate.swift:258-291). Contraction is off in these
+// functions, so each operation rounds as in a CPU evaluation.
+
+GARG_DEV void plane_position(const PlaneRay64& s, const double e1[3], const double e2[3], double out[3]) {
+#pragma clang fp contract(off)
+    const double c = cos(s.phi), sn = sin(s.phi);
+    for (int k = 0; k < 3; ++k) out[k] = s.r * (c * e1[k] + sn * e2[k]);
+}
+
+// The direction a static observer at the ray's point sees (Schwarzschild.swift:58-67).
+GARG_DEV void plane_direction(const PlaneRay64& s, const double e1[3], const double e2[3], double out[3]) {
+#pragma clang fp contract(off)
+    const double c = cos(s.phi), sn = sin(s.phi);
+    const double f = fmax(1.0 - R_SCHWARZSCHILD / s.r, 1e-6);
+    const double radial_rate = s.rdot / sqrt(f), tangential_rate = s.r * s.phidot;
+    double local[3];
+    for (int k = 0; k < 3; ++k) local[k] = radial_rate * (c * e1[k] + sn * e2[k]) + tangential_rate * (-sn * e1[k] + c * e2[k]);
+    const double len = sqrt(local[0] * local[0] + local[1] * local[1] + local[2] * local[2]);
+    for (int k = 0; k < 3; ++k) out[k] = len > 0.0 ? local[k] / len : c * e1[k] + sn * e2[k];
+}
+
+// VolumeField.sample in double precision over the resolved fp16 volume.
+GARG_DEV void field_sample64(const Render64& a, const double p[3], double e[4], double v[3]) {
+#pragma clang fp contract(off)
+    const int n = (int)a.size;
+    const double half[3] = {a.half_xy, a.half_xy, a.half_z};
+    double frac[3];
+    int base[3];
+    for (int k = 0; k < 3; ++k) {
+        const double unit = (p[k] + half[k]) / (2.0 * half[k]);
+        const double t = unit * (double)n - 0.5;
+        const double b = floor(t);
+        frac[k] = t - b;
+        base[k] = (int)b;
+    }
+    for (int c = 0; c < 4; ++c) e[c] = 0.0;
+    for (int c = 0; c < 3; ++c) v[c] = 0.0;
+    for (int dz = 0; dz <= 1; ++dz) {
+        const double wz = dz == 0 ? 1.0 - frac[2] : frac[2];
+        for (int dy = 0; dy <= 1; ++dy) {
+            const double wy = dy == 0 ? 1.0 - frac[1] : frac[1];
+            for (int dx = 0; dx <= 1; ++dx) {
+                const double wx = dx == 0 ? 1.0 - frac[0] : frac[0];
+                const double w = wx * wy * wz;
+                if (w == 0.0) continue;
+                const int x = base[0] + dx, y = base[1] + dy, z = base[2] + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= n || y >= n || z >= n) continue;
+                const Voxel& vox = a.voxels[((size_t)z * (size_t)n + (size_t)y) * (size_t)n + (size_t)x];
+                for (int c = 0; c < 4; ++c) e[c] += (double)vox.emission[c] * w;
+                for (int c = 0; c < 3; ++c) v[c] += (double)vox.velocity[c] * w;
+            }
+        }
+    }
+}
+
+// Schwarzschild.swift:230-240: the static observer's Doppler and
+// gravitational factor for gas of coordinate speed |gas|, capped at 0.85 c.
+GARG_DEV double redshift_factor64(double f, const double direction[3], const double gas[3]) {
+#pragma clang fp contract(off)
+    const double magnitude = sqrt(gas[0] * gas[0] + gas[1] * gas[1] + gas[2] * gas[2]);
+    const double speed = fmin(magnitude, 8.0);
+    const double beta = speed > 0.0 ? GAS_SPEED_CEILING * tanh(speed / GAS_SPEED_CEILING) : 0.0;
+    double along = 0.0;
+    for (int k = 0; k < 3; ++k) along += direction[k] * (magnitude > 0.0 ? gas[k] * (beta / magnitude) : 0.0);
+    const double gamma = 1.0 / sqrt(1.0 - beta * beta);
+    return sqrt(fmax(f, 0.0)) / (gamma * (1.0 + along));
+}
+
+__global__ __launch_bounds__(64) void render_fp64_kernel(Render64 a) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    if (i >= a.width * a.height) return;
+    const uint32_t px = i % a.width, py = i / a.width;
+    // The pixel's camera ray (LensedRender.swift:23-28).
+    const double x = (2.0 * ((double)px + 0.5) / (double)a.width - 1.0) * a.tan_half_fov[0];
+    const double y = (1.0 - 2.0 * ((double)py + 0.5) / (double)a.height) * a.tan_half_fov[1];
+    double d[3];
+    for (int k = 0; k < 3; ++k) d[k] = a.forward[k] + x * a.right[k] + y * a.up[k];
+    const double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int k = 0; k < 3; ++k) d[k] = d[k] / dl;
+    double e1[3], e2[3];
+    PlaneRay64 s = plane_launch(a.position, d, e1, e2);
+    double prev[3], color[3] = {0.0, 0.0, 0.0}, transmittance = 1.0;
+    plane_position(s, e1, e2, prev);
+    const ProbeSettings64& g = a.settings;
+    for (uint64_t k = 0; k < g.step_cap; ++k) {
+        const double coarse = fmin(fmax(g.empty_step_factor * s.r, g.step_min), g.empty_step_max);
+        const double h = fabs(prev[2]) > DISK_SLAB_HALF_HEIGHT + coarse
+                             ? coarse
+                             : fmin(fmax(g.step_factor * s.r, g.step_min), g.step_max);
+        plane_rk4(s, h);
+        double pos[3];
+        plane_position(s, e1, e2, pos);
+        if (s.r <= g.capture_radius) break;
+        if (fabs(pos[2]) < DISK_SLAB_HALF_HEIGHT && fabs(pos[0]) < a.half_xy && fabs(pos[1]) < a.half_xy &&
+            transmittance > TRANSMITTANCE_FLOOR) {
+            double e[4], v[3];
+            field_sample64(a, pos, e, v);
+            if (e[3] > 0.0) {
+                const double dx = pos[0] - prev[0], dy = pos[1] - prev[1], dz = pos[2] - prev[2];
+                const double path = sqrt(dx * dx + dy * dy + dz * dz);
+                double g3 = 1.0;
+                if (a.redshift) {
+                    double dir[3];
+                    plane_direction(s, e1, e2, dir);
+                    const double f = 1.0 - R_SCHWARZSCHILD / s.r;
+                    const double gr = redshift_factor64(f, dir, v);
+                    const double gravity = sqrt(fmax(f, 0.0));
+                    const double bulk = fmin(e[3] / BEAMING_DENSITY_FLOOR, 1.0);
+                    g3 = gravity * gravity * gravity * (1.0 - bulk) + gr * gr * gr * bulk;
+                }
+                const double alpha = 1.0 - exp(-a.opacity * e[3] * path);
+                const double weight = transmittance * g3 * path;
+                for (int c = 0; c < 3; ++c) color[c] = color[c] + weight * e[c];
+                transmittance *= 1.0 - alpha;
+            }
+        }
+        for (int c = 0; c < 3; ++c) prev[c] = pos[c];
+        if (s.r >= g.escape_radius) break;
+    }
+    for (int c = 0; c < 3; ++c) a.rgb[3 * (size_t)i + c] = color[c];
+}
+
+hipError_t launch_render_fp64(const Render64& a, hipStream_t stream) {
+    render_fp64_kernel<<<dim3((a.width * a.height + 63u) / 64u), 64, 0, stream>>>(a);
+    return hipGetLastError();
 }
 
 hipError_t launch_probe_fp64(const garg_ray_probe* probes, uint32_t count, const ProbeSettings64& g,

@@ -4,12 +4,12 @@
 //
 // The product never links the FP64 CPU oracle. Parity and sky, whose bar
 // is the reference's FP64 integrator, are judged against the library's FP64
-// GPU run of that integrator (garg_probe_fp64); beaming, whose bar is a full
-// FP64 render with the gas, is left to the test-side runner (python
-// tests/validate.py), which drives the same C ABI. Every other golden is
-// judged here: closed forms (shadow, deflection), budgets (conservation),
-// bitwise (determinism), the disk edge fit (isco), GPU against GPU (jump),
-// and shadow_kerr against the GPU's FP64 bisection.
+// GPU run of that integrator (garg_probe_fp64), and beaming, whose bar is
+// the Oracle's FP64 lensed render with the gas, against the library's FP64
+// GPU render (garg_render_fp64). The other goldens are judged by closed
+// forms (shadow, deflection), budgets (conservation), bitwise equality
+// (determinism), the disk edge fit (isco), GPU against GPU (jump), and
+// shadow_kerr against the GPU's FP64 bisection.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -307,9 +307,58 @@ void run_validate(const Options& o) {
         rows.push_back({g, fmt("max relative endpoint error %.2e, %.0f outcome mismatches", worst, mismatched),
                         g.passes(worst) && mismatched == 0 ? "pass" : "FAIL"});
     }
-    // beaming: needs a full FP64 render with the gas, which only the
-    // test-side oracle has; measured and judged by tests/validate.py.
-    rows.push_back({load(dir, "beaming"), "needs the FP64 CPU render", "oracle"});
+    // beaming (Validate.swift:258-291): the brightness ratio of the image
+    // halves of the GPU's still march against that of the Oracle's FP64
+    // lensed render of the same resolved volume, here the library's FP64 GPU
+    // render (garg_render_fp64).
+    {
+        Golden g = load(dir, "beaming");
+        const int w = (int)g.p("width"), h = (int)g.p("height");
+        OrbitCamera cam;
+        garg_march_uniforms u = make_uniforms(cam, w, h, Policy::still, true, 0, false, 0.0, true);
+        garg_targets* t = nullptr;
+        check(garg_targets_create(s.ctx, w, h, &t), "allocating targets");
+        check(garg_march_image(s.ctx, nullptr, &u, s.volume, nullptr, t, 0, 0, GARG_INTEGRATOR_REFERENCE), "marching");
+        std::vector<uint16_t> color((size_t)w * h * 4);
+        check(garg_targets_read(t, color.data(), nullptr, nullptr), "reading");
+        garg_targets_destroy(t);
+        std::vector<double> lum((size_t)w * h), oracle_lum((size_t)w * h);
+        for (size_t i = 0; i < lum.size(); ++i) {
+            double rgb[3];
+            for (int k = 0; k < 3; ++k) rgb[k] = half_to_float(color[4 * i + k]);
+            lum[i] = 0.2126 * rgb[0] + 0.7152 * rgb[1] + 0.0722 * rgb[2];
+        }
+        garg_render_camera_fp64 oc{};
+        float position[3], right[3], up[3], forward[3];
+        cam.basis(position, right, up, forward);
+        for (int k = 0; k < 3; ++k) {
+            oc.position[k] = position[k];
+            oc.right[k] = right[k];
+            oc.up[k] = up[k];
+            oc.forward[k] = forward[k];
+        }
+        const double tan_half = std::tan((double)cam.fov_y * 0.5);
+        oc.tan_half_fov[0] = tan_half * w / h;
+        oc.tan_half_fov[1] = tan_half;
+        const garg_geodesic_settings still = policy_settings(Policy::still);
+        std::vector<double> rgb((size_t)w * h * 3);
+        check(garg_render_fp64(s.ctx, nullptr, s.volume, &oc, (uint32_t)w, (uint32_t)h, &still, VOLUME_OPACITY, 1u,
+                               rgb.data()),
+              "rendering in FP64");
+        for (size_t i = 0; i < oracle_lum.size(); ++i)
+            oracle_lum[i] = 0.2126 * rgb[3 * i] + 0.7152 * rgb[3 * i + 1] + 0.0722 * rgb[3 * i + 2];
+        // LensedRender.sideRatio (LensedRender.swift:99-109): the brighter
+        // half over the dimmer, split at the column through the hole.
+        auto side_ratio = [&](const std::vector<double>& l) {
+            double left = 0, right_sum = 0;
+            for (int y = 0; y < h; ++y)
+                for (int x = 0; x < w; ++x) (x < w / 2 ? left : right_sum) += l[(size_t)y * w + x];
+            const double high = std::max(left, right_sum), low = std::min(left, right_sum);
+            return low > 0.0 ? high / low : INFINITY;
+        };
+        const double gpu = side_ratio(lum), fp64 = side_ratio(oracle_lum), m = gpu / fp64;
+        rows.push_back({g, fmt("gpu %.3f vs fp64 %.3f, ratio %.3f", gpu, fp64, m), g.passes(m) ? "pass" : "FAIL"});
+    }
     // sky (Validate.swift:293-320): the sweep tables' asymptotic directions
     // against the FP64 integrator run out to oracleRadius.
     {
@@ -421,7 +470,6 @@ void run_validate(const Options& o) {
         line(buf);
         if (r.status == "FAIL") all = false;
     }
-    line("beaming is judged against the FP64 CPU render by: python tests/validate.py");
     s.close();
     exit(all ? 0 : 1);
 }

@@ -402,6 +402,26 @@ typedef struct {
 int garg_probe_fp64(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* probes, uint32_t count,
                     const garg_geodesic_settings* settings, garg_ray_probe_fp64_result* results);
 
+/* The reference Oracle's lensed render (Oracle/LensedRender.swift:46-91:
+ * LensedRender.Camera, trace, image) on the GPU in double precision: every
+ * pixel's camera ray integrated with the FP64 integrator of
+ * garg_probe_fp64 and composited through the resolved volume, sampled as
+ * VolumeField.sample does (VolumeField.swift:40-63) from the fp16 fields
+ * widened to double. The reference's beaming golden compares the GPU march's
+ * image with this render (Validate.swift:258-291); `gargantua validate`
+ * judges it against this. `rgb` receives 3 doubles per pixel, row major,
+ * linear and without background. Synchronous; at most 2^22 pixels. */
+typedef struct {
+    double position[3];
+    double right[3];
+    double up[3];
+    double forward[3];
+    double tan_half_fov[2]; /* tan(fov / 2) scaled by the aspect in x, and in y */
+} garg_render_camera_fp64;
+int garg_render_fp64(garg_ctx* ctx, garg_stream* stream, const garg_volume* volume,
+                     const garg_render_camera_fp64* camera, uint32_t width, uint32_t height,
+                     const garg_geodesic_settings* settings, double opacity, uint32_t redshift, double* rgb);
+
 /* ---- measurement ------------------------------------------------------------ */
 
 /* No counterpart in the reference: the ceiling the splat bin is measured
@@ -410,6 +430,12 @@ int garg_probe_fp64(garg_ctx* ctx, garg_stream* stream, const garg_ray_probe* pr
  * and returns the rate. Synchronous. */
 int garg_atomic_rate(garg_ctx* ctx, garg_stream* stream, uint32_t words, uint32_t adds_per_thread, uint32_t threads,
                      double* adds_per_second);
+
+/* No counterpart in the reference: the FP32 ceiling the march is measured
+ * against. Times `threads` x `iterations` x 32 independent non-packed FP32
+ * fused multiply-adds (2 flops each) and returns flops per second. Synchronous. */
+int garg_valu_rate(garg_ctx* ctx, garg_stream* stream, uint32_t iterations, uint32_t threads,
+                   double* flops_per_second);
 
 /* ---- graphs --------------------------------------------------------------- */
 

@@ -4,6 +4,8 @@ import os
 import subprocess
 import sys
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -36,3 +38,18 @@ def test_autotune_brick_table_round_trip(tmp_path, monkeypatch):
     assert autotune.brick_for("gfx950:test", "splat_1M_128") == (32, 8, 4)
     autotune.save("gfx950:test", "splat_4M_256", {"brick": [4, 4, 4]}, 0.5)
     assert autotune.brick_for("gfx950:test", "splat_4M_256") == api.Volume.BRICKS[0]
+
+
+@pytest.mark.gpu
+def test_measured_ceilings_are_physical(ctx):
+    """The ceilings bench.py states beside its roofline: non-packed FP32
+    fused multiply-adds cannot exceed MI355X's FP32 vector peak (157.3
+    TFLOP/s) and reach at least a fifth of the 78.6 TFLOP/s a wave64 FMA
+    every four cycles per SIMD gives; no-return atomics run at a positive
+    rate. Bad arguments are refused, not run."""
+    tflops = ctx.valu_rate(256, 1 << 18) / 1e12
+    assert 15.0 < tflops < 157.3
+    assert ctx.atomic_rate(1 << 20, 4, 1 << 16) > 0.0
+    from gargantua_amd import api
+    with pytest.raises(api.GargError):
+        ctx.valu_rate(0, 1 << 18)

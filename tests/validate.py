@@ -1,12 +1,12 @@
 #!/usr/bin/env python3
 """The full golden suite on this GPU, judged against the FP64 CPU oracle.
 
-`gargantua validate` judges ten of the eleven goldens itself (parity and sky
-against the library's FP64 GPU integrator); beaming needs an FP64 render
-with the gas, which only the oracle has. This runner covers all eleven
-(Validation/Validate.swift:17), each against the CPU oracle, by running the
-golden tests of tests/ on the device and printing one row per golden. Exit 0 when every
-golden passes, 1 otherwise. Test infrastructure: it imports the oracle.
+`gargantua validate` judges all eleven goldens in the product (parity and
+sky against the library's FP64 GPU integrator, beaming against its FP64 GPU
+render). This runner judges the same eleven (Validation/Validate.swift:17)
+against the CPU oracle instead, by running the golden tests of tests/ on
+the device and printing one row per golden. Exit 0 when every golden
+passes, 1 otherwise. Test infrastructure: it imports the oracle.
 """
 import os
 import re
