@@ -244,7 +244,10 @@ def main():
         progress("plain launches")
 
     achieved = steps_per_frame * FLOPS_PER_KERR_STEP / (ms_per_step * 1e-3) / 1e12
-    fma_ceiling = max(ctx.valu_rate() for _ in range(3)) / 1e12
+    fma_forms = {name: max(ctx.valu_rate(form) for _ in range(3)) / 1e12
+                 for name, form in (("literal_operand", api.VALU_FMA_LITERAL), ("three_vgprs", api.VALU_FMA_VGPR),
+                                    ("sgpr_operands", api.VALU_FMA_SGPR))}
+    fma_ceiling = fma_forms["literal_operand"]
     line = {
         "metric": METRIC,
         "value": value,
@@ -280,9 +283,11 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
                      "traffic_unit": "bytes per launch",
-                     # The FP32 ceiling measured on this GPU in the same run:
-                     # independent non-packed fused multiply-adds (garg_valu_rate).
-                     "fma_ceiling_measured": fma_ceiling,
+                     # FP32 ceilings measured on this GPU in the same run:
+                     # independent non-packed fused multiply-adds in three
+                     # operand forms (garg_valu_rate), TFLOP/s; the fastest
+                     # form is the achievable FP32 ceiling.
+                     "fma_measured_tflops": fma_forms,
                      "frac_of_measured": achieved / fma_ceiling if fma_ceiling else None,
                      "flops_per_step": FLOPS_PER_KERR_STEP,
                      "kernel": ("march_tiles_kernel" if cfg.refill_threshold == 0 else "march_batch_kernel")

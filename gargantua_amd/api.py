@@ -31,6 +31,7 @@ INTEGRATOR_REFERENCE, INTEGRATOR_FAST, INTEGRATOR_KERR_SCHILD, INTEGRATOR_POLAR 
 GARG_E_ARGUMENT, GARG_E_DEVICE, GARG_E_BUDGET, GARG_E_STATE = 1, 2, 3, 4
 FRAME_AUTO = 0xFFFFFFFF  # splat frame from the particle store's dispatch count
 PROBE_NOALLOC_MAX = 16384  # GARG_PROBE_NOALLOC_MAX: probes per garg_probe_noalloc call
+VALU_FMA_LITERAL, VALU_FMA_VGPR, VALU_FMA_SGPR = 0, 1, 2  # garg_valu_rate operand forms
 
 
 class GargError(RuntimeError):
@@ -163,7 +164,7 @@ SIGNATURES = {
     "garg_shadow_boundary": (C.c_int, [vp, vp, f64, f64, f64, u32, u32, u32, u32, vp, vp]),
     "garg_particles_dispatches": (C.c_int, [vp, P(u32)]),
     "garg_atomic_rate": (C.c_int, [vp, vp, u32, u32, u32, P(f64)]),
-    "garg_valu_rate": (C.c_int, [vp, vp, u32, u32, P(f64)]),
+    "garg_valu_rate": (C.c_int, [vp, vp, u32, u32, u32, P(f64)]),
     "garg_graph_begin": (C.c_int, [vp, vp]),
     "garg_graph_end": (C.c_int, [vp, vp, P(vp)]),
     "garg_graph_launch": (C.c_int, [vp, vp]),
@@ -303,10 +304,11 @@ class Context:
         _check(self.lib.garg_atomic_rate(self.h, None, words, adds_per_thread, threads, C.byref(r)))
         return r.value
 
-    def valu_rate(self, iterations: int = 2048, threads: int = 1 << 21) -> float:
-        """Non-packed FP32 fused multiply-add flops per second (garg_valu_rate)."""
+    def valu_rate(self, form: int = 0, iterations: int = 2048, threads: int = 1 << 21) -> float:
+        """Non-packed FP32 fused multiply-add flops per second in one operand
+        form (VALU_FMA_LITERAL, VALU_FMA_VGPR, VALU_FMA_SGPR; garg_valu_rate)."""
         r = f64()
-        _check(self.lib.garg_valu_rate(self.h, None, iterations, threads, C.byref(r)))
+        _check(self.lib.garg_valu_rate(self.h, None, form, iterations, threads, C.byref(r)))
         return r.value
 
     def graph_begin(self, stream=None):

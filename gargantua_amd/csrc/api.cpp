@@ -1465,9 +1465,10 @@ int garg_atomic_rate(garg_ctx* ctx, garg_stream* stream, uint32_t words, uint32_
     return GARG_OK;
 }
 
-int garg_valu_rate(garg_ctx* ctx, garg_stream* stream, uint32_t iterations, uint32_t threads,
+int garg_valu_rate(garg_ctx* ctx, garg_stream* stream, uint32_t form, uint32_t iterations, uint32_t threads,
                    double* flops_per_second) {
     REQUIRE(ctx && flops_per_second, "ctx and flops_per_second are required.");
+    REQUIRE(form <= GARG_VALU_FMA_SGPR, "unknown FMA operand form %u.", form);
     REQUIRE(iterations > 0 && threads > 0, "iterations and threads must be positive.");
     REFUSE_WHILE_CAPTURING(ctx, stream, "garg_valu_rate");
     hipStream_t s = stream_of(ctx, stream);
@@ -1478,9 +1479,9 @@ int garg_valu_rate(garg_ctx* ctx, garg_stream* stream, uint32_t iterations, uint
     hipEvent_t a = nullptr, b = nullptr;
     hipError_t e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
-    if (e == hipSuccess) e = launch_valu_rate((float*)buffer, threads, iterations, s);  // warm
+    if (e == hipSuccess) e = launch_valu_rate((float*)buffer, form, threads, iterations, s);  // warm
     if (e == hipSuccess) e = hipEventRecord(a, s);
-    if (e == hipSuccess) e = launch_valu_rate((float*)buffer, threads, iterations, s);
+    if (e == hipSuccess) e = launch_valu_rate((float*)buffer, form, threads, iterations, s);
     if (e == hipSuccess) e = hipEventRecord(b, s);
     if (e == hipSuccess) e = hipEventSynchronize(b);
     float ms = 0.0f;

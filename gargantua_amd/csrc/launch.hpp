@@ -170,8 +170,9 @@ hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, co
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
 hipError_t launch_fill_zero(void* ptr, uint64_t bytes, hipStream_t stream);
 hipError_t launch_atomic_rate(uint32_t* buffer, uint32_t words, uint32_t adds, uint32_t threads, hipStream_t stream);
-// `threads` x `iterations` x 32 independent FP32 fused multiply-adds.
-hipError_t launch_valu_rate(float* out, uint32_t threads, uint32_t iterations, hipStream_t stream);
+// `threads` x `iterations` x 32 independent FP32 fused multiply-adds in
+// operand form `form` (0: two VGPRs and a literal, 1: three VGPRs, 2: SGPRs).
+hipError_t launch_valu_rate(float* out, uint32_t form, uint32_t threads, uint32_t iterations, hipStream_t stream);
 
 // FP64 sweep tables: table[i] for the sweep from `radius` over b at table
 // coordinate i / (n - 1) between low and high; kind 0 = sweep to infinity,

@@ -213,24 +213,33 @@ hipError_t launch_probe(const ProbeArgs& a, uint32_t integrator, hipStream_t str
     return hipGetLastError();
 }
 
-// The FP32 ceiling the march is measured against: every thread runs
-// `iterations` rounds of 32 independent fused multiply-adds (one non-packed
-// v_fma_f32 each: packed FP32 instructions are disabled in this build) and
-// stores their sum, so that none of them is dead code. Thirty-two chains
-// and a twice-unrolled loop leave the issue of the FMAs as the only bound.
+// The FP32 ceilings the march is measured against: every thread runs
+// `iterations` rounds of 32 independent non-packed fused multiply-adds
+// (packed FP32 instructions are disabled in this build) and stores their
+// sum, so that none of them is dead code. The operand form sets the issue
+// rate on gfx950: FORM 0 reads two VGPRs and a literal (v_fmamk_f32), FORM 1
+// three VGPRs (v_fma_f32), FORM 2 a VGPR and two SGPRs (the wave-uniform
+// kernel arguments).
 constexpr int VALU_RATE_CHAINS = 32;
 
+template <int FORM>
 __global__ __launch_bounds__(256) void valu_rate_kernel(float* out, uint32_t threads, uint32_t iterations, float m,
                                                        float c) {
     uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= threads) return;
+    // Per-lane multiplier and addend (VGPRs) for FORM 1, close to m and c.
+    const float mv = m + (float)(t & 7u) * 1e-9f, cv = c + (float)(t & 3u) * 1e-12f;
     float x[VALU_RATE_CHAINS];
 #pragma unroll
     for (int j = 0; j < VALU_RATE_CHAINS; ++j) x[j] = (float)(t & 1023u) * 1e-3f + (float)j;
 #pragma unroll 2
     for (uint32_t i = 0; i < iterations; ++i) {
 #pragma unroll
-        for (int j = 0; j < VALU_RATE_CHAINS; ++j) x[j] = __builtin_fmaf(x[j], m, c);
+        for (int j = 0; j < VALU_RATE_CHAINS; ++j) {
+            if constexpr (FORM == 0) x[j] = __builtin_fmaf(x[j], 0.99999994f, cv);
+            else if constexpr (FORM == 1) x[j] = __builtin_fmaf(x[j], mv, cv);
+            else x[j] = __builtin_fmaf(x[j], m, c);
+        }
     }
     float s = 0.0f;
 #pragma unroll
@@ -238,8 +247,11 @@ __global__ __launch_bounds__(256) void valu_rate_kernel(float* out, uint32_t thr
     out[t] = s;
 }
 
-hipError_t launch_valu_rate(float* out, uint32_t threads, uint32_t iterations, hipStream_t stream) {
-    valu_rate_kernel<<<dim3((threads + 255u) / 256u), 256, 0, stream>>>(out, threads, iterations, 0.99999994f, 1e-7f);
+hipError_t launch_valu_rate(float* out, uint32_t form, uint32_t threads, uint32_t iterations, hipStream_t stream) {
+    const dim3 grid((threads + 255u) / 256u);
+    if (form == 0) valu_rate_kernel<0><<<grid, 256, 0, stream>>>(out, threads, iterations, 0.99999994f, 1e-7f);
+    else if (form == 1) valu_rate_kernel<1><<<grid, 256, 0, stream>>>(out, threads, iterations, 0.99999994f, 1e-7f);
+    else valu_rate_kernel<2><<<grid, 256, 0, stream>>>(out, threads, iterations, 0.99999994f, 1e-7f);
     return hipGetLastError();
 }
 

@@ -431,10 +431,14 @@ int garg_render_fp64(garg_ctx* ctx, garg_stream* stream, const garg_volume* volu
 int garg_atomic_rate(garg_ctx* ctx, garg_stream* stream, uint32_t words, uint32_t adds_per_thread, uint32_t threads,
                      double* adds_per_second);
 
-/* No counterpart in the reference: the FP32 ceiling the march is measured
+/* No counterpart in the reference: the FP32 ceilings the march is measured
  * against. Times `threads` x `iterations` x 32 independent non-packed FP32
- * fused multiply-adds (2 flops each) and returns flops per second. Synchronous. */
-int garg_valu_rate(garg_ctx* ctx, garg_stream* stream, uint32_t iterations, uint32_t threads,
+ * fused multiply-adds (2 flops each) in one operand form and returns flops
+ * per second. The form sets the issue rate: two VGPRs and a literal
+ * (v_fmamk_f32), three VGPRs (v_fma_f32), or SGPR operands (wave-uniform
+ * values). Synchronous. */
+enum { GARG_VALU_FMA_LITERAL = 0, GARG_VALU_FMA_VGPR = 1, GARG_VALU_FMA_SGPR = 2 };
+int garg_valu_rate(garg_ctx* ctx, garg_stream* stream, uint32_t form, uint32_t iterations, uint32_t threads,
                    double* flops_per_second);
 
 /* ---- graphs --------------------------------------------------------------- */

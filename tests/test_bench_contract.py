@@ -43,13 +43,16 @@ def test_autotune_brick_table_round_trip(tmp_path, monkeypatch):
 @pytest.mark.gpu
 def test_measured_ceilings_are_physical(ctx):
     """The ceilings bench.py states beside its roofline: non-packed FP32
-    fused multiply-adds cannot exceed MI355X's FP32 vector peak (157.3
-    TFLOP/s) and reach at least a fifth of the 78.6 TFLOP/s a wave64 FMA
-    every four cycles per SIMD gives; no-return atomics run at a positive
-    rate. Bad arguments are refused, not run."""
-    tflops = ctx.valu_rate(256, 1 << 18) / 1e12
-    assert 15.0 < tflops < 157.3
-    assert ctx.atomic_rate(1 << 20, 4, 1 << 16) > 0.0
+    fused multiply-adds in each operand form cannot exceed MI355X's FP32
+    vector peak (157.3 TFLOP/s) and reach at least a fifth of the 78.6
+    TFLOP/s of a wave64 FMA every four cycles per SIMD; no-return atomics run
+    at a positive rate. Bad arguments are refused, not run."""
     from gargantua_amd import api
+    for form in (api.VALU_FMA_LITERAL, api.VALU_FMA_VGPR, api.VALU_FMA_SGPR):
+        tflops = ctx.valu_rate(form, 256, 1 << 18) / 1e12
+        assert 15.0 < tflops <= 157.3, (form, tflops)
+    assert ctx.atomic_rate(1 << 20, 4, 1 << 16) > 0.0
     with pytest.raises(api.GargError):
-        ctx.valu_rate(0, 1 << 18)
+        ctx.valu_rate(api.VALU_FMA_LITERAL, 0, 1 << 18)
+    with pytest.raises(api.GargError):
+        ctx.valu_rate(3, 16, 1 << 18)
