@@ -696,16 +696,21 @@ GARG_DEV KerrEval kerr_eval_sc(const KerrConsts& k, float r, float st, float ct,
     float G = (k.L2 * inv_sin2 - k.two_aEL) + k.a2E2 * sin2;
     float pr2 = pr * pr, pth2 = pth * pth;
     float F = ((delta * pr2 + pth2) - P * P_over_delta) + G;
-    float H = 0.5f * F * inv_sigma;
-    float delta_prime = 2.0f * r - 2.0f;
-    float fr = (delta_prime * pr2 - 4.0f * r * k.E * P_over_delta) + P_over_delta * P_over_delta * delta_prime;
+    // The factors 1/2, 2 and 4 of H = F / (2 Sigma), dDelta/dr = 2 (r - 1)
+    // and the r and theta derivatives of F ride on the half-quantities
+    // below: scaling by a power of two is exact in float, so each product
+    // equals the one with the factor written out, with fewer instructions.
+    float twice_H = F * inv_sigma;
+    float half_delta_prime = r - 1.0f;
+    float half_fr = (half_delta_prime * pr2 - 2.0f * r * k.E * P_over_delta) +
+                    P_over_delta * P_over_delta * half_delta_prime;
     float sc = e.st * e.ct;
-    float fth = -2.0f * k.L2 * e.ct * inv_sin2 * inv_ms + 2.0f * k.a2E2 * sc;
+    float half_fth = -k.L2 * e.ct * inv_sin2 * inv_ms + k.a2E2 * sc;
     e.dr = delta * pr * inv_sigma;
     e.dth = pth * inv_sigma;
     e.dph = ((k.a * P_over_delta + k.L * inv_sin2) - k.aE) * inv_sigma;
-    e.dpr = (2.0f * r * H - 0.5f * fr) * inv_sigma;
-    e.dpth = (-2.0f * k.a2 * sc * H - 0.5f * fth) * inv_sigma;
+    e.dpr = (r * twice_H - half_fr) * inv_sigma;
+    e.dpth = (-k.a2 * sc * twice_H - half_fth) * inv_sigma;
     e.null_residual = F;
     e.carter = pth2 + (e.ct * e.ct) * (k.L2 * inv_sin2 - k.a2E2);
     e.inv_sigma = inv_sigma;
