@@ -54,7 +54,8 @@ LAYOUT_PROBE = r"""
 int main(void) {
     S(garg_particle) S(garg_geodesic_settings) S(garg_march_uniforms) S(garg_ray_probe)
     S(garg_ray_probe_result) S(garg_march_counters) S(garg_device_info) S(garg_march_request) S(garg_batch_config)
-    S(garg_ray_probe_fp64_result)
+    S(garg_ray_probe_fp64_result) S(garg_render_camera_fp64)
+    F(garg_render_camera_fp64, forward) F(garg_render_camera_fp64, tan_half_fov)
     F(garg_march_uniforms, tile_origin) F(garg_march_uniforms, star_seed) F(garg_march_uniforms, sphere_radius)
     F(garg_march_uniforms, spin) F(garg_march_uniforms, volume_half_extent_z) F(garg_march_uniforms, geodesic)
     F(garg_march_counters, steps) F(garg_march_request, targets) F(garg_march_request, ray_begin)
@@ -75,7 +76,8 @@ def test_record_layouts_match_the_header():
           "garg_march_uniforms": api.MarchUniforms, "garg_ray_probe": api.RayProbe,
           "garg_ray_probe_result": api.RayProbeResult, "garg_march_counters": api.MarchCounters,
           "garg_device_info": api.DeviceInfo, "garg_march_request": api.MarchRequest,
-          "garg_batch_config": api.BatchConfig, "garg_ray_probe_fp64_result": api.RayProbeFp64Result}
+          "garg_batch_config": api.BatchConfig, "garg_ray_probe_fp64_result": api.RayProbeFp64Result,
+          "garg_render_camera_fp64": api.RenderCameraFp64}
     for name, cls in py.items():
         assert C.sizeof(cls) == c[name], name
     for key, value in c.items():

@@ -344,6 +344,18 @@ def test_fp64_render_matches_the_cpu_oracle(ctx, settled_volume):
     assert abs(ra - rb) / rb < 1e-9
 
 
+def test_fp64_render_refuses_bad_arguments(ctx, settled_volume):
+    """Sizes beyond the render's bound and a camera inside the capture
+    radius are refused on the host with a sentence, before any launch."""
+    s = api.settings("still")
+    with pytest.raises(api.GargError, match="1 to 2"):
+        ctx.render_fp64(settled_volume, api.orbit_camera(), 0, 10, s)
+    with pytest.raises(api.GargError, match="1 to 2"):
+        ctx.render_fp64(settled_volume, api.orbit_camera(), 4096, 2048, s)
+    with pytest.raises(api.GargError, match="capture radius"):
+        ctx.render_fp64(settled_volume, api.orbit_camera(distance=1.5), 8, 8, s)
+
+
 def test_jump_golden(ctx, settled_volume):
     g = load_golden("jump")
     w, h = int(g["parameters"]["width"]), int(g["parameters"]["height"])
