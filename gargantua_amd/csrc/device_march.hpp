@@ -1390,6 +1390,21 @@ GARG_DEV KsState ks_from_bl_f64(double a, const KerrRayState& st) {
                    (float)(vz + f * lz * lk)};
 }
 
+// A ray's Kerr-Schild state where it leaves the Boyer-Lindquist form, and
+// the log of |r - r+| / |r - r-| there, in double precision. Not inlined:
+// a ray passes here at most once, and kept apart its double-precision
+// constants and temporaries do not stay live in the march's registers.
+struct PolarEntry {
+    KsState state;
+    float log_ratio;
+};
+
+__device__ __attribute__((noinline)) PolarEntry polar_entry_f64(float a, KerrRayState st) {
+    const double root = sqrt(fmax(1.0 - (double)a * a, 0.0));
+    const double ratio = fabs(((double)st.r - (1.0 + root)) / ((double)st.r - (1.0 - root)));
+    return PolarEntry{ks_from_bl_f64(a, st), (float)log(ratio)};
+}
+
 // The ray is kept in a KerrSchildMarch. While it is in the Boyer-Lindquist
 // form that struct's float fields hold the KerrFastMarch fields instead
 // (unpack/pack below, plain value copies the compiler keeps in the same
@@ -1483,10 +1498,8 @@ struct KerrHybridMarch {
     // From the Boyer-Lindquist form to the Kerr-Schild form at the current point.
     GARG_DEV void to_polar(const KerrFastMarch& m) {
         const KerrRayState st{m.r, m.th, m.ph, m.pr, m.pth, m.k.E, m.k.L};
-        const KsState state = ks_from_bl_f64(m.k.a, st);
-        const double root = sqrt(fmax(1.0 - (double)m.k.a * m.k.a, 0.0));
-        const double ratio = fabs(((double)st.r - (1.0 + root)) / ((double)st.r - (1.0 - root)));
-        ks.resume(m.k.a, st.energy, state, (float)log(ratio), m.carter0, m.k.L, m.inv_scale, m.g, m.drift, m.steps);
+        const PolarEntry entry = polar_entry_f64(m.k.a, st);
+        ks.resume(m.k.a, st.energy, entry.state, entry.log_ratio, m.carter0, m.k.L, m.inv_scale, m.g, m.drift, m.steps);
         polar = true;
     }
 
