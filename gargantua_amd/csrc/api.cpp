@@ -888,6 +888,7 @@ int garg_march_image(garg_ctx* ctx, garg_stream* stream, const garg_march_unifor
     a.debug = targets->debug;
     a.depth = targets->depth;
     a.counters = targets->counters;
+    a.totals = nullptr;
     a.extent_w = extent_w ? std::min(extent_w, uniforms->resolution[0] - uniforms->tile_origin[0])
                           : uniforms->resolution[0] - uniforms->tile_origin[0];
     a.extent_h = extent_h ? std::min(extent_h, uniforms->resolution[1] - uniforms->tile_origin[1])
@@ -1046,6 +1047,39 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
             members[m] = i;
             member_rays[m++] = rays;
             max_cap = std::max(max_cap, r.uniforms.geodesic.step_cap);
+        }
+        // With whole-tile claims, a lone request of a whole image at most two
+        // and a half rounds of waves deep goes to the one-launch-per-frame
+        // kernel instead: its one-wave workgroups fit more waves per SIMD
+        // than the persistent tile loop, which pays only over more rounds
+        // (or more requests). The rays take the same steps and give the same
+        // bits either way. A lane-refill configuration always runs the
+        // persistent kernel.
+        if (m == 1u && config->refill_threshold == 0u) {
+            const garg_march_request& r = requests[members[0]];
+            const uint32_t integrator = !kerr                       ? GARG_INTEGRATOR_FAST
+                                        : config->kerr_schild == 1u ? GARG_INTEGRATOR_KERR_SCHILD
+                                        : config->kerr_schild == 2u ? GARG_INTEGRATOR_POLAR
+                                                                    : GARG_INTEGRATOR_FAST;
+            const uint64_t slots = (uint64_t)march_image_resident_blocks(integrator) * ctx->info.compute_units;
+            const uint64_t tiles = garg_request_rays(r.uniforms.resolution[0], r.uniforms.resolution[1]) / 64u;
+            if (r.ray_begin == 0u &&
+                member_rays[0] == garg_request_rays(r.uniforms.resolution[0], r.uniforms.resolution[1]) &&
+                r.uniforms.tile_origin[0] == 0u && r.uniforms.tile_origin[1] == 0u && slots > 0u &&
+                2u * tiles <= 5u * slots) {
+                MarchImageArgs a;
+                a.uniforms = device_uniforms(&r.uniforms, r.volume);
+                a.scene = make_scene(ctx, r.volume, kerr ? nullptr : r.tables);
+                a.color = r.targets->color;
+                a.debug = r.targets->debug;
+                a.depth = r.targets->depth;
+                a.counters = r.targets->counters;
+                a.totals = ctx->batch_totals;
+                a.extent_w = r.uniforms.resolution[0];
+                a.extent_h = r.uniforms.resolution[1];
+                HIP_TRY(launch_march_image(a, integrator, s), "launching marchImage for a lone request");
+                continue;
+            }
         }
         // Each request is queued as up to BATCH_BANDS bands of whole tiles
         // (horizontal bands of its image), the middle bands first and the

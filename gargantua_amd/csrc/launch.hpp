@@ -17,6 +17,7 @@ struct MarchImageArgs {
     h4* debug;
     float* depth;
     garg_march_counters* counters;
+    garg_march_counters* totals;  // also counted here when not null (a batch's totals)
     uint32_t extent_w, extent_h;
 };
 
@@ -29,6 +30,9 @@ struct ProbeArgs {
 };
 
 hipError_t launch_march_image(const MarchImageArgs& a, uint32_t integrator, hipStream_t stream);
+// Workgroups (one wave each) of the one-launch-per-frame march that one
+// compute unit holds at once, for `integrator`; 0 if unknown.
+uint32_t march_image_resident_blocks(uint32_t integrator);
 hipError_t launch_probe(const ProbeArgs& a, uint32_t integrator, hipStream_t stream);
 
 // One request of the persistent march as the device sees it.

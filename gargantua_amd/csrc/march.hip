@@ -102,6 +102,9 @@ __global__ __launch_bounds__(64) void march_image_kernel(MarchImageArgs a) {
     }
     add_counters(a.counters, active, r.drift > u.drift_budget, r.outcome == GARG_RAY_EXHAUSTED,
                  r.outcome == GARG_RAY_CAPTURED, r.steps);
+    if (a.totals)
+        add_counters(a.totals, active, r.drift > u.drift_budget, r.outcome == GARG_RAY_EXHAUSTED,
+                     r.outcome == GARG_RAY_CAPTURED, r.steps);
 }
 
 template <int Form>
@@ -198,6 +201,23 @@ hipError_t launch_march_image(const MarchImageArgs& a, uint32_t integrator, hipS
     else
         march_image_kernel<GARG_INTEGRATOR_FAST><<<grid, 64, 0, stream>>>(a);
     return hipGetLastError();
+}
+
+template <int Form>
+static uint32_t image_resident_blocks() {
+    int n = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, march_image_kernel<Form>, 64, 0) == hipSuccess && n > 0
+               ? (uint32_t)n
+               : 0u;
+}
+
+uint32_t march_image_resident_blocks(uint32_t integrator) {
+    switch (integrator) {
+        case GARG_INTEGRATOR_REFERENCE: return image_resident_blocks<GARG_INTEGRATOR_REFERENCE>();
+        case GARG_INTEGRATOR_KERR_SCHILD: return image_resident_blocks<GARG_INTEGRATOR_KERR_SCHILD>();
+        case GARG_INTEGRATOR_POLAR: return image_resident_blocks<GARG_INTEGRATOR_POLAR>();
+        default: return image_resident_blocks<GARG_INTEGRATOR_FAST>();
+    }
 }
 
 hipError_t launch_probe(const ProbeArgs& a, uint32_t integrator, hipStream_t stream) {

@@ -335,9 +335,12 @@ int garg_batch_config_default(garg_ctx* ctx, garg_batch_config* out);
  * garg_batch_config): one launch for the spinning requests
  * and one for the Schwarzschild ones, at most 256 requests. Each request is
  * queued as up to four bands of whole tiles, middle bands first and
- * interleaved across requests; no result depends on the queue order. Each
- * request's targets counters accumulate; `total` (optional) receives the
- * sums over the call, read back synchronously. */
+ * interleaved across requests; no result depends on the queue order. With
+ * whole-tile claims, a lone request of a whole image at most two and a half
+ * rounds of waves deep takes the one-launch-per-frame kernel instead, whose
+ * workgroups fit more waves per SIMD (same bits). Each request's targets
+ * counters accumulate; `total` (optional) receives the sums over the call,
+ * read back synchronously. */
 int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_request* requests, uint32_t count,
                      const garg_batch_config* config, garg_march_counters* total);
 

@@ -195,7 +195,7 @@ def test_batched_kerr_schild_matches_plain_bitwise(ctx, threshold, form):
     the same per-ray state machine as the plain Kerr-Schild launch: every
     target bit-identical, for whole-tile claims and for lane refill, alone
     and with a second request of another spin, size and budget in the mix."""
-    from tests.test_gpu_march import disk_volume, kerr_request, render_plain
+    from tests.test_gpu_march import disk_volume, halves, kerr_request, render_plain
     vol = disk_volume(ctx)
     a = kerr_request(ctx, 256, 144, 0.9, 60.0, 50.0, 4096, vol)
     b = kerr_request(ctx, 120, 72, 0.5, 80.0, 40.0, 256, vol, compensated=False)
@@ -203,7 +203,7 @@ def test_batched_kerr_schild_matches_plain_bitwise(ctx, threshold, form):
     plain_b = render_plain(ctx, b, form)[0]
     cfg = ctx.batch_config(compaction_period=8, refill_threshold=threshold,
                            kerr_schild=1 if form == api.INTEGRATOR_KERR_SCHILD else 2)
-    ctx.march_batch([a], cfg)
+    ctx.march_batch(halves(a), cfg)
     assert same_bits(plain_a, a["targets"].read())
     ctx.march_batch([b, a], cfg)
     assert same_bits(plain_a, a["targets"].read()) and same_bits(plain_b, b["targets"].read())

@@ -52,6 +52,15 @@ def render_plain(ctx, req, integrator=api.INTEGRATOR_FAST):
     return out
 
 
+def halves(req):
+    """One request as two batch entries (two ray ranges of the same image):
+    a batch of two always runs the persistent kernel, where a lone small
+    image with whole-tile claims goes to the one-launch kernel."""
+    rays = api.request_rays(req["targets"].width, req["targets"].height)
+    half = (rays // 2) & ~63
+    return [dict(req, ray_begin=0, ray_count=half), dict(req, ray_begin=half, ray_count=rays - half)]
+
+
 def same_bits(a, b):
     return all(np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x, y.view(np.uint32)
                               if y.dtype == np.float32 else y) for x, y in zip(a, b))
@@ -115,7 +124,7 @@ def test_batched_matches_plain_bitwise_kerr(ctx, threshold):
     vol = disk_volume(ctx)
     req = kerr_request(ctx, 256, 144, 0.9, 60.0, 50.0, 4096, vol)
     (plain, pc) = render_plain(ctx, req)
-    total = ctx.march_batch([req], ctx.batch_config(compaction_period=8, refill_threshold=threshold))
+    total = ctx.march_batch(halves(req), ctx.batch_config(compaction_period=8, refill_threshold=threshold))
     batched = req["targets"].read()
     assert same_bits(plain, batched)
     bc = req["targets"].counters()
@@ -133,9 +142,34 @@ def test_batched_matches_plain_bitwise_schwarzschild_with_jump(ctx, threshold, c
     tables = api.Tables(ctx)
     req = schw_request(ctx, 192, 108, vol, tables, cap)
     (plain, pc) = render_plain(ctx, req)
-    ctx.march_batch([req], ctx.batch_config(compaction_period=16, refill_threshold=threshold))
+    ctx.march_batch(halves(req), ctx.batch_config(compaction_period=16, refill_threshold=threshold))
     assert same_bits(plain, req["targets"].read())
     assert req["targets"].counters().steps == pc.steps
+
+
+@pytest.mark.parametrize("kind", ["kerr", "schwarzschild"])
+def test_lone_small_request_takes_the_one_launch_kernel(ctx, kind):
+    """With whole-tile claims a lone whole image of a few rounds of waves
+    goes to the one-launch kernel inside garg_march_batch: the same bits as
+    the plain launch, and its counters in the request's targets and in the
+    batch's totals as the persistent kernel would count them."""
+    vol = disk_volume(ctx)
+    tables = api.Tables(ctx)
+    req = (kerr_request(ctx, 256, 144, 0.9, 60.0, 50.0, 4096, vol) if kind == "kerr"
+           else schw_request(ctx, 192, 108, vol, tables))
+    (plain, pc) = render_plain(ctx, req)
+    req["targets"].reset_counters()
+    total = ctx.march_batch([req], ctx.batch_config(refill_threshold=0))
+    assert same_bits(plain, req["targets"].read())
+    bc = req["targets"].counters()
+    for c in (bc, total):
+        assert (c.rays, c.steps, c.captured, c.exhausted, c.over_drift_budget) == \
+            (pc.rays, pc.steps, pc.captured, pc.exhausted, pc.over_drift_budget)
+    # The same image as two batch entries runs the persistent kernel: same bits, same totals.
+    req["targets"].reset_counters()
+    total2 = ctx.march_batch(halves(req), ctx.batch_config(refill_threshold=0))
+    assert same_bits(plain, req["targets"].read()) and total2.steps == pc.steps and total2.rays == pc.rays
+    req["targets"].close()
 
 
 @pytest.mark.parametrize("block,period,threshold", [(64, 1, 64), (128, 32, 16), (256, 64, 0)])
