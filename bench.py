@@ -103,6 +103,15 @@ def time_frames(ctx, api, run, warmup, steps):
     return total_ms, frames
 
 
+def settled_ms(ctx, api, run, reps, warmup=1):
+    """Time per call of a side leg, timed twice with the second pass kept:
+    a leg of frames of a millisecond or two would otherwise be charged for
+    the clocks settling after the leg before it, whichever leg runs first."""
+    time_frames(ctx, api, run, warmup, reps)
+    ms, _ = time_frames(ctx, api, run, 0, reps)
+    return ms / reps
+
+
 def host_threads():
     """Cores this process may use: those it is allowed to run on, within the
     thread count the host grants it (OMP_NUM_THREADS: a GPU box gives one
@@ -352,13 +361,12 @@ def schwarzschild_legs(ctx, api, default_volume, cfg, cpu: bool):
         totals = ctx.march_batch(req, cfg)
         steps = int(totals.steps)
         reps = 3 if cap > 256 else 10
-        batched_ms, frames = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, req, cfg), 1, reps)
+        batched_ms = settled_ms(ctx, api, lambda: _batch_nosync(ctx, api, req, cfg), reps)
         leg = {"rays": int(totals.rays), "steps": steps, "over_drift_budget": int(totals.over_drift_budget),
-               "exhausted": int(totals.exhausted), "batched_ms": batched_ms / reps,
-               "batched_steps_per_s": steps / (batched_ms / reps * 1e-3)}
+               "exhausted": int(totals.exhausted), "batched_ms": batched_ms,
+               "batched_steps_per_s": steps / (batched_ms * 1e-3)}
         for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
-            ms, _ = time_frames(ctx, api, lambda: ctx.march_image(u, t, vol, tables, integrator), 1, reps)
-            leg[form + "_ms"] = ms / reps
+            leg[form + "_ms"] = settled_ms(ctx, api, lambda: ctx.march_image(u, t, vol, tables, integrator), reps)
         leg["speedup_vs_plain_reference"] = leg["plain_reference_ms"] / leg["batched_ms"]
         if cpu and name.startswith("small"):
             import oracle as O
@@ -400,16 +408,15 @@ def kerr_views_leg(ctx, api, vol, cfg, views=8):
         r["targets"].reset_counters()
     totals = ctx.march_batch(reqs, cfg)
     steps = int(totals.steps)
-    batched_ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, reqs, cfg), 1, 10)
-    out = {"requests": views, "rays": int(totals.rays), "steps": steps, "batched_ms": batched_ms / 10,
-           "batched_steps_per_s": steps / (batched_ms / 10 * 1e-3), "batched_form": _form_name(cfg)}
+    batched_ms = settled_ms(ctx, api, lambda: _batch_nosync(ctx, api, reqs, cfg), 10)
+    out = {"requests": views, "rays": int(totals.rays), "steps": steps, "batched_ms": batched_ms,
+           "batched_steps_per_s": steps / (batched_ms * 1e-3), "batched_form": _form_name(cfg)}
     for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST),
                              ("plain_polar", api.INTEGRATOR_POLAR)):
         def run_all():
             for r in reqs:
                 ctx.march_image(r["uniforms"], r["targets"], vol, None, integrator)
-        ms, _ = time_frames(ctx, api, run_all, 1, 5)
-        out[form + "_ms"] = ms / 5
+        out[form + "_ms"] = settled_ms(ctx, api, run_all, 5)
     out["speedup_vs_plain_reference"] = out["plain_reference_ms"] / out["batched_ms"]
     out["speedup_vs_plain_fast"] = out["plain_fast_ms"] / out["batched_ms"]
     out["speedup_vs_plain_polar"] = out["plain_polar_ms"] / out["batched_ms"]
@@ -437,13 +444,12 @@ def kerr_presets_leg(ctx, api, vol, cfg):
         req = [{"uniforms": u, "targets": t, "volume": vol}]
         t.reset_counters()
         totals = ctx.march_batch(req, cfg)
-        ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, req, cfg), 1, 10)
-        leg = {"rays": int(totals.rays), "steps": int(totals.steps), "batched_ms": ms / 10,
+        leg = {"rays": int(totals.rays), "steps": int(totals.steps),
+               "batched_ms": settled_ms(ctx, api, lambda: _batch_nosync(ctx, api, req, cfg), 10),
                "batched_form": _form_name(cfg)}
         for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST),
                                  ("plain_polar", api.INTEGRATOR_POLAR)):
-            pms, _ = time_frames(ctx, api, lambda: ctx.march_image(u, t, vol, None, integrator), 1, 5)
-            leg[form + "_ms"] = pms / 5
+            leg[form + "_ms"] = settled_ms(ctx, api, lambda: ctx.march_image(u, t, vol, None, integrator), 5)
         leg["speedup_vs_plain_reference"] = leg["plain_reference_ms"] / leg["batched_ms"]
         leg["speedup_vs_plain_polar"] = leg["plain_polar_ms"] / leg["batched_ms"]
         t.close()
@@ -463,19 +469,19 @@ def bake_leg(ctx, api, vol, tables):
     tables.apply(u)
     bake = api.Bake(ctx, w, h, 96)
     t = api.Targets(ctx, w, h)
-    bake_ms, _ = time_frames(ctx, api, lambda: bake.rows(u, tables), 1, 5)
-    walk_ms, _ = time_frames(ctx, api, lambda: bake.walk(u, vol, t), 1, 5)
+    bake_ms = settled_ms(ctx, api, lambda: bake.rows(u, tables), 5)
+    walk_ms = settled_ms(ctx, api, lambda: bake.walk(u, vol, t), 5)
     headers, _ = bake.read()
     # The walk's algorithmic HBM bytes per ray: the 16-byte header, 8 bytes
     # per stored sample it reads (at most the ray's count; it stops at the
     # transmittance floor) and 12 bytes written (colour and depth). The
     # volume's texels are L2-resident reads, not counted.
     walk_bytes = int(w * h * (16 + 12) + 8 * int(headers[:, 0].sum()))
-    out = {"bake_ms": bake_ms / 5, "walk_ms": walk_ms / 5, "rays": w * h, "overflow_rays": bake.overflow(),
+    out = {"bake_ms": bake_ms, "walk_ms": walk_ms, "rays": w * h, "overflow_rays": bake.overflow(),
            "mean_samples_per_ray": float(headers[:, 0].mean()),
            "bake_bytes": w * h * (8 * 96 + 16),
-           "walk_bytes_upper": walk_bytes, "walk_GB_per_s": walk_bytes / (walk_ms / 5 * 1e-3) / 1e9,
-           "walk_hbm_frac": walk_bytes / (walk_ms / 5 * 1e-3) / 8e12,
+           "walk_bytes_upper": walk_bytes, "walk_GB_per_s": walk_bytes / (walk_ms * 1e-3) / 1e9,
+           "walk_hbm_frac": walk_bytes / (walk_ms * 1e-3) / 8e12,
            # The bake writes its samples and headers: 8 bytes per stored sample, 16 per ray.
            "bake_written_bytes": int(16 * w * h + 8 * int(headers[:, 0].sum()))}
     bake.close()
@@ -500,15 +506,14 @@ def small_batch_leg(ctx, api, vol, tables, cfg, views=8):
         r["targets"].reset_counters()
     totals = ctx.march_batch(reqs, cfg)
     steps = int(totals.steps)
-    batched_ms, _ = time_frames(ctx, api, lambda: _batch_nosync(ctx, api, reqs, cfg), 1, 10)
-    out = {"requests": views, "rays": int(totals.rays), "steps": steps, "batched_ms": batched_ms / 10,
-           "batched_steps_per_s": steps / (batched_ms / 10 * 1e-3)}
+    batched_ms = settled_ms(ctx, api, lambda: _batch_nosync(ctx, api, reqs, cfg), 10)
+    out = {"requests": views, "rays": int(totals.rays), "steps": steps, "batched_ms": batched_ms,
+           "batched_steps_per_s": steps / (batched_ms * 1e-3)}
     for form, integrator in (("plain_reference", api.INTEGRATOR_REFERENCE), ("plain_fast", api.INTEGRATOR_FAST)):
         def run_all():
             for r in reqs:
                 ctx.march_image(r["uniforms"], r["targets"], vol, tables, integrator)
-        ms, _ = time_frames(ctx, api, run_all, 1, 10)
-        out[form + "_ms"] = ms / 10
+        out[form + "_ms"] = settled_ms(ctx, api, run_all, 10)
     out["speedup_vs_plain_reference"] = out["plain_reference_ms"] / out["batched_ms"]
     out["speedup_vs_plain_fast"] = out["plain_fast_ms"] / out["batched_ms"]
     for r in reqs:

@@ -93,7 +93,10 @@ struct garg_ctx {
     hipEvent_t batch_done = nullptr;        // the previous call's launches have run
     unsigned long long* queue = nullptr;
     garg_march_counters* batch_totals = nullptr;
-    garg_march_counters* scratch_counters = nullptr;
+    // Workgroups per compute unit of the one-launch march, per integrator
+    // (0 = not queried yet), for garg_march_batch's lone-request choice.
+    uint32_t image_resident[4] = {0, 0, 0, 0};
+    garg_march_counters* lone_counters = nullptr;  // a lone request's counts in garg_march_batch
     void* probe_scratch = nullptr;
     uint64_t probe_scratch_bytes = 0;
 };
@@ -300,7 +303,7 @@ int garg_ctx_create(int device, uint64_t budget_bytes, garg_ctx** out) {
     HIP_TRY(hipEventCreateWithFlags(&ctx->batch_done, hipEventDisableTiming), "creating an event");
     if ((rc = ctx_alloc(ctx, 2 * 8 * 16 * sizeof(unsigned long long), (void**)&ctx->queue, "the ray queue"))) return rc;
     if ((rc = ctx_alloc(ctx, sizeof(garg_march_counters), (void**)&ctx->batch_totals, "batch counters"))) return rc;
-    if ((rc = ctx_alloc(ctx, sizeof(garg_march_counters), (void**)&ctx->scratch_counters, "counters"))) return rc;
+    if ((rc = ctx_alloc(ctx, sizeof(garg_march_counters), (void**)&ctx->lone_counters, "counters"))) return rc;
     *out = ctx;
     return GARG_OK;
 }
@@ -314,7 +317,7 @@ int garg_ctx_destroy(garg_ctx* ctx) {
     ctx_free(ctx, ctx->batch_requests, 2 * sizeof(BatchRequest) * GARG_BATCH_MAX_REQUESTS);
     ctx_free(ctx, ctx->queue, 2 * 8 * 16 * sizeof(unsigned long long));
     ctx_free(ctx, ctx->batch_totals, sizeof(garg_march_counters));
-    ctx_free(ctx, ctx->scratch_counters, sizeof(garg_march_counters));
+    ctx_free(ctx, ctx->lone_counters, sizeof(garg_march_counters));
     ctx_free(ctx, ctx->probe_scratch, ctx->probe_scratch_bytes);
     if (ctx->batch_staging) (void)hipHostFree(ctx->batch_staging);
     for (hipEvent_t e : ctx->batch_staging_free)
@@ -888,7 +891,6 @@ int garg_march_image(garg_ctx* ctx, garg_stream* stream, const garg_march_unifor
     a.debug = targets->debug;
     a.depth = targets->depth;
     a.counters = targets->counters;
-    a.totals = nullptr;
     a.extent_w = extent_w ? std::min(extent_w, uniforms->resolution[0] - uniforms->tile_origin[0])
                           : uniforms->resolution[0] - uniforms->tile_origin[0];
     a.extent_h = extent_h ? std::min(extent_h, uniforms->resolution[1] - uniforms->tile_origin[1])
@@ -1061,7 +1063,8 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
                                         : config->kerr_schild == 1u ? GARG_INTEGRATOR_KERR_SCHILD
                                         : config->kerr_schild == 2u ? GARG_INTEGRATOR_POLAR
                                                                     : GARG_INTEGRATOR_FAST;
-            const uint64_t slots = (uint64_t)march_image_resident_blocks(integrator) * ctx->info.compute_units;
+            if (!ctx->image_resident[integrator]) ctx->image_resident[integrator] = march_image_resident_blocks(integrator);
+            const uint64_t slots = (uint64_t)ctx->image_resident[integrator] * ctx->info.compute_units;
             const uint64_t tiles = garg_request_rays(r.uniforms.resolution[0], r.uniforms.resolution[1]) / 64u;
             if (r.ray_begin == 0u &&
                 member_rays[0] == garg_request_rays(r.uniforms.resolution[0], r.uniforms.resolution[1]) &&
@@ -1073,11 +1076,13 @@ int garg_march_batch(garg_ctx* ctx, garg_stream* stream, const garg_march_reques
                 a.color = r.targets->color;
                 a.debug = r.targets->debug;
                 a.depth = r.targets->depth;
-                a.counters = r.targets->counters;
-                a.totals = ctx->batch_totals;
+                a.counters = ctx->lone_counters;
                 a.extent_w = r.uniforms.resolution[0];
                 a.extent_h = r.uniforms.resolution[1];
+                HIP_TRY(launch_fill_zero(ctx->lone_counters, sizeof(garg_march_counters), s), "clearing counters");
                 HIP_TRY(launch_march_image(a, integrator, s), "launching marchImage for a lone request");
+                HIP_TRY(launch_add_counters(r.targets->counters, ctx->batch_totals, ctx->lone_counters, s),
+                        "adding the request's counters");
                 continue;
             }
         }

@@ -17,7 +17,6 @@ struct MarchImageArgs {
     h4* debug;
     float* depth;
     garg_march_counters* counters;
-    garg_march_counters* totals;  // also counted here when not null (a batch's totals)
     uint32_t extent_w, extent_h;
 };
 
@@ -30,6 +29,9 @@ struct ProbeArgs {
 };
 
 hipError_t launch_march_image(const MarchImageArgs& a, uint32_t integrator, hipStream_t stream);
+// Adds the counters at `src` to those at `a` and at `b` (one thread).
+hipError_t launch_add_counters(garg_march_counters* a, garg_march_counters* b, const garg_march_counters* src,
+                               hipStream_t stream);
 // Workgroups (one wave each) of the one-launch-per-frame march that one
 // compute unit holds at once, for `integrator`; 0 if unknown.
 uint32_t march_image_resident_blocks(uint32_t integrator);

@@ -102,9 +102,29 @@ __global__ __launch_bounds__(64) void march_image_kernel(MarchImageArgs a) {
     }
     add_counters(a.counters, active, r.drift > u.drift_budget, r.outcome == GARG_RAY_EXHAUSTED,
                  r.outcome == GARG_RAY_CAPTURED, r.steps);
-    if (a.totals)
-        add_counters(a.totals, active, r.drift > u.drift_budget, r.outcome == GARG_RAY_EXHAUSTED,
-                     r.outcome == GARG_RAY_CAPTURED, r.steps);
+}
+
+// One launch's counters added to a request's counters and a batch's totals:
+// a batch that gives a lone request to the one-launch kernel counts it into
+// a block of its own, so that the kernel's waves add to one place only.
+__global__ void add_counters_kernel(garg_march_counters* a, garg_march_counters* b, const garg_march_counters* src) {
+    if (threadIdx.x != 0) return;
+    const garg_march_counters c = *src;
+    garg_march_counters* const dst[2] = {a, b};
+    for (int k = 0; k < 2; ++k) {
+        uint32_t* d = reinterpret_cast<uint32_t*>(dst[k]);
+        atomicAdd(&d[0], c.rays);
+        atomicAdd(&d[1], c.over_drift_budget);
+        atomicAdd(&d[2], c.exhausted);
+        atomicAdd(&d[3], c.captured);
+        atomicAdd((unsigned long long*)&dst[k]->steps, (unsigned long long)c.steps);
+    }
+}
+
+hipError_t launch_add_counters(garg_march_counters* a, garg_march_counters* b, const garg_march_counters* src,
+                               hipStream_t stream) {
+    add_counters_kernel<<<1, 64, 0, stream>>>(a, b, src);
+    return hipGetLastError();
 }
 
 template <int Form>
