@@ -648,8 +648,10 @@ def pass_leg(ctx, api, particles, vol):
 def max_preset_passes(ctx, api, timed):
     """The serving disk state (4M particles, 256^3): one sim dispatch (64
     bytes per particle), the stochastic splat that serving runs every frame
-    (one corner of six no-return atomics per particle, 28 bytes read) and
-    the resolve with its fused clear (64 bytes per voxel)."""
+    (one corner per particle: sorted by voxel brick, the default, and the
+    reference's pass of six no-return atomics per particle; 28 bytes of
+    particle read either way) and the resolve with its fused clear (64
+    bytes per voxel)."""
     n, size = 4_000_000, 256
     voxels = size ** 3
     p = api.Particles(ctx, n, seed=1)
@@ -665,14 +667,23 @@ def max_preset_passes(ctx, api, timed):
         v.splat(p, exact=False, frame=frame[0])
 
     splat_ms = timed(stochastic, 20)
+    v.method = api.SPLAT_ATOMIC
+    atomic_ms = timed(stochastic, 20)
+    v.method = api.SPLAT_SORTED
     resolve_ms = timed(lambda: v.resolve(clear=True), 20)
     ceiling = ctx.atomic_rate(6 * voxels, 6, n)
     out = {"sim": {"ms": sim_ms, "bytes": 64 * n, "GB_per_s": 64 * n / (sim_ms * 1e-3) / 1e9,
                    "hbm_frac": 64 * n / (sim_ms * 1e-3) / 8e12},
-           "splat_stochastic": {"ms": splat_ms, "deposits_per_s": n / (splat_ms * 1e-3),
-                                "atomics_per_s": 6 * n / (splat_ms * 1e-3), "atomic_ceiling_per_s": ceiling,
-                                "vs_atomic_ceiling": 6 * n / (splat_ms * 1e-3) / ceiling,
-                                "bytes_read": 28 * n, "GB_per_s_read": 28 * n / (splat_ms * 1e-3) / 1e9},
+           # Sorted by brick: count, column scan, scatter of 32-byte records
+           # and one LDS gather per brick, no global atomics; against it the
+           # atomic pass and the rate of the same count of atomics.
+           "splat_stochastic": {"ms": splat_ms, "method": "counting sort by 16x16x8 brick",
+                                "deposits_per_s": n / (splat_ms * 1e-3),
+                                "bytes_read": 28 * n, "GB_per_s_read": 28 * n / (splat_ms * 1e-3) / 1e9,
+                                "speedup_vs_atomic_pass": atomic_ms / splat_ms,
+                                "atomic_pass": {"ms": atomic_ms, "atomics_per_s": 6 * n / (atomic_ms * 1e-3),
+                                                "atomic_ceiling_per_s": ceiling,
+                                                "vs_atomic_ceiling": 6 * n / (atomic_ms * 1e-3) / ceiling}},
            "resolve": {"ms": resolve_ms, "bytes": 64 * voxels, "GB_per_s": 64 * voxels / (resolve_ms * 1e-3) / 1e9,
                        "hbm_frac": 64 * voxels / (resolve_ms * 1e-3) / 8e12}}
     v.close()

@@ -32,6 +32,7 @@ GARG_E_ARGUMENT, GARG_E_DEVICE, GARG_E_BUDGET, GARG_E_STATE = 1, 2, 3, 4
 FRAME_AUTO = 0xFFFFFFFF  # splat frame from the particle store's dispatch count
 PROBE_NOALLOC_MAX = 16384  # GARG_PROBE_NOALLOC_MAX: probes per garg_probe_noalloc call
 VALU_FMA_LITERAL, VALU_FMA_VGPR, VALU_FMA_SGPR = 0, 1, 2  # garg_valu_rate operand forms
+SPLAT_SORTED, SPLAT_ATOMIC = 0, 1  # garg_volume_set_splat_method
 
 
 class GargError(RuntimeError):
@@ -128,6 +129,8 @@ SIGNATURES = {
     "garg_volume_clear": (C.c_int, [vp, vp]),
     "garg_volume_set_splat_brick": (C.c_int, [vp, u32, u32, u32]),
     "garg_volume_splat_brick": (C.c_int, [vp, P(u32)]),
+    "garg_volume_set_splat_method": (C.c_int, [vp, u32]),
+    "garg_volume_splat_method": (C.c_int, [vp, P(u32)]),
     "garg_splat_bin": (C.c_int, [vp, vp, vp, u32, u32]),
     "garg_splat_resolve": (C.c_int, [vp, vp, u32]),
     "garg_volume_read_sums": (C.c_int, [vp, vp, vp, vp]),
@@ -524,6 +527,18 @@ class Volume:
     @brick.setter
     def brick(self, shape):
         _check(self.ctx.lib.garg_volume_set_splat_brick(self.h, *[int(v) for v in shape]))
+
+    @property
+    def method(self) -> int:
+        """SPLAT_SORTED (the default) or SPLAT_ATOMIC: how splat() deposits;
+        both give the same sums."""
+        m = u32()
+        _check(self.ctx.lib.garg_volume_splat_method(self.h, C.byref(m)))
+        return m.value
+
+    @method.setter
+    def method(self, method: int):
+        _check(self.ctx.lib.garg_volume_set_splat_method(self.h, int(method)))
 
     def splat(self, particles: Particles, exact: bool = True, frame: int = 0, stream=None):
         _check(self.ctx.lib.garg_splat_bin(self.h, particles.h, stream, 1 if exact else 0, frame))

@@ -137,6 +137,10 @@ struct garg_volume {
     void* gather_base;  // the exact deposit's cell starts and brick stamps
     uint64_t gather_bytes;
     SplatGather gather;
+    void* bins_base;  // the one-corner deposit's brick tables (null past its largest side)
+    uint64_t bins_bytes;
+    SplatBins bins;
+    uint32_t splat_method;  // GARG_SPLAT_SORTED or GARG_SPLAT_ATOMIC
 };
 
 struct garg_tables {
@@ -585,6 +589,23 @@ int garg_volume_create(garg_ctx* ctx, uint32_t size, float half_extent, float ha
         delete v;
         return rc;
     }
+    const uint64_t bin_words = splat_bin_table_words(size);
+    v->bins = splat_bin_layout(size);
+    v->bins_base = nullptr;
+    v->bins_bytes = align256(bin_words * 4u);
+    if (bin_words) {
+        rc = ctx_alloc(ctx, v->bins_bytes, &v->bins_base, "the deposit's brick tables");
+        if (rc) {
+            ctx_free(ctx, v->base, v->bytes);
+            ctx_free(ctx, v->gather_base, v->gather_bytes);
+            delete v;
+            return rc;
+        }
+        v->bins.counts = (uint32_t*)v->bins_base;
+        v->bins.totals = v->bins.counts + (uint64_t)kSplatBinGroups * v->bins.bricks;
+        v->bins.starts = v->bins.totals + v->bins.bricks;
+    }
+    v->splat_method = GARG_SPLAT_SORTED;
     v->gather.cells = (uint32_t*)v->gather_base;
     v->gather.ends = (uint32_t*)((char*)v->gather_base + cells);
     v->gather.bricks = (uint32_t*)((char*)v->gather_base + 2u * cells);
@@ -596,6 +617,7 @@ int garg_volume_create(garg_ctx* ctx, uint32_t size, float half_extent, float ha
     if (e != hipSuccess) {
         ctx_free(ctx, v->base, v->bytes);
         ctx_free(ctx, v->gather_base, v->gather_bytes);
+        if (v->bins_base) ctx_free(ctx, v->bins_base, v->bins_bytes);
         delete v;
         return fail(GARG_E_DEVICE, "clearing the volume: %s", hipGetErrorString(e));
     }
@@ -609,6 +631,7 @@ int garg_volume_destroy(garg_volume* v) {
     (void)hipStreamSynchronize(v->ctx->stream);
     ctx_free(v->ctx, v->base, v->bytes);
     ctx_free(v->ctx, v->gather_base, v->gather_bytes);
+    if (v->bins_base) ctx_free(v->ctx, v->bins_base, v->bins_bytes);
     delete v;
     return GARG_OK;
 }
@@ -639,16 +662,38 @@ int garg_splat_bin(garg_volume* v, garg_particles* p, garg_stream* stream, uint3
     a.seed_hi = (uint32_t)(p->seed >> 32);
     a.frame_counter = frame == GARG_FRAME_AUTO ? p->view.dispatches : nullptr;
     hipStream_t s = stream_of(v->ctx, stream);
-    // The exact deposit sorts and gathers (the same integer sums, no
-    // atomics). Inside a graph capture it deposits directly: the library
-    // sort clears its storage with memset nodes, which graphs must not hold.
-    if (a.exact && !capturing(s)) {
+    // The sorted deposits give the atomic deposit's integer sums without its
+    // device-scope atomics. The exact one sorts with the library's radix
+    // sort, which clears its storage with memset nodes that graphs must not
+    // hold, so inside a capture it deposits directly; the one-corner deposit
+    // sorts by brick with kernels only, captured or not, on volumes whose
+    // bricks its tables hold.
+    const bool sorted = v->splat_method == GARG_SPLAT_SORTED;
+    if (sorted && a.exact && !capturing(s)) {
         // A new stamp per deposit; zero is the cleared bricks' value.
         if (++v->gather.stamp == 0u) v->gather.stamp = 1u;
         HIP_TRY(launch_splat_sorted(a, p->sort, v->gather, s), "launching the sorted splat");
+    } else if (sorted && !a.exact && v->bins.bricks != 0u) {
+        SplatBins bins = v->bins;
+        bins.keys = p->sort.keys_in;
+        bins.records = reinterpret_cast<uint4*>(p->sort.records);
+        HIP_TRY(launch_splat_binned(a, bins, s), "launching the binned splat");
     } else {
         HIP_TRY(launch_splat_bin(a, s), "launching splatParticles");
     }
+    return GARG_OK;
+}
+
+int garg_volume_set_splat_method(garg_volume* v, uint32_t method) {
+    REQUIRE(v, "volume is required.");
+    REQUIRE(method <= GARG_SPLAT_ATOMIC, "unknown splat method %u (0 sorted, 1 atomic).", method);
+    v->splat_method = method;
+    return GARG_OK;
+}
+
+int garg_volume_splat_method(garg_volume* v, uint32_t* method) {
+    REQUIRE(v && method, "volume and method are required.");
+    *method = v->splat_method;
     return GARG_OK;
 }
 

@@ -173,6 +173,29 @@ uint64_t splat_gather_bricks(uint32_t size, const uint32_t shape[3]);
 uint64_t splat_gather_bricks_max(uint32_t size);
 hipError_t launch_splat_sorted(const SplatArgs& a, const SplatSortBuffers& b, const SplatGather& g,
                                hipStream_t stream);
+// The one-corner deposit by a counting sort over 16 x 16 x 8 voxel bricks
+// (splat_sorted.hip): per-particle keys and two 16-byte records (the
+// particle store's sort buffers), and per-volume tables of
+// kSplatBinGroups x bricks counts, the bricks' totals and their bricks + 1
+// starts. Volumes of more than kSplatBinMaxBricks bricks (sides past 320
+// voxels) deposit directly.
+constexpr uint32_t kSplatBinBrick[3] = {16, 16, 8};
+constexpr uint32_t kSplatBinMaxBricks = 16384;
+constexpr uint32_t kSplatBinGroups = 256;
+struct SplatBins {
+    uint32_t* keys;
+    uint4* records;
+    uint32_t* counts;
+    uint32_t* totals;
+    uint32_t* starts;
+    uint32_t bricks, nbx, nby;  // bricks = 0: the volume is too large for this path
+    uint32_t groups, chunk;     // set at launch from the particle count
+};
+// The brick grid of a volume side (bricks = 0 past kSplatBinMaxBricks), and
+// the words its tables need (0 then).
+SplatBins splat_bin_layout(uint32_t size);
+uint64_t splat_bin_table_words(uint32_t size);
+hipError_t launch_splat_binned(const SplatArgs& a, SplatBins bins, hipStream_t stream);
 hipError_t launch_resolve(const ResolveArgs& a, hipStream_t stream);
 hipError_t launch_fill_zero(void* ptr, uint64_t bytes, hipStream_t stream);
 hipError_t launch_atomic_rate(uint32_t* buffer, uint32_t words, uint32_t adds, uint32_t threads, hipStream_t stream);

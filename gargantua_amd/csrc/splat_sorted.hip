@@ -324,6 +324,251 @@ __global__ __launch_bounds__(BRICK_THREADS) void splat_brick_kernel(SplatArgs a,
     }
 }
 
+// ---------------------------------------------------------------------------
+// The one-corner (stochastic) deposit by a counting sort over voxel bricks.
+//
+// Each particle lands in one voxel, so the particles are grouped by the
+// 16 x 16 x 8 brick of that voxel with a counting sort of the project's own:
+// every workgroup of the count pass histograms its chunk of particles by
+// brick in LDS, a column scan turns the histograms into each workgroup's
+// offset within each brick, and the scatter pass writes each particle's
+// integer contribution (a 32-byte record) at its brick's next free slot.
+// One workgroup per brick then sums its records in an LDS box and adds the
+// box to the brick's sums, as the exact deposit's gather does. Every buffer
+// is written in full before it is read, so the four kernels hold no memset
+// and replay inside captured graphs. Integer sums modulo 2^32 make the
+// result bit-identical to splat_kernel's atomics whatever the order.
+
+namespace {
+
+constexpr uint32_t BIN_X = kSplatBinBrick[0], BIN_Y = kSplatBinBrick[1], BIN_Z = kSplatBinBrick[2];
+constexpr uint32_t BIN_VOXELS = BIN_X * BIN_Y * BIN_Z;
+constexpr uint32_t BIN_THREADS = 1024;
+constexpr uint32_t BIN_SEGMENTS = 16;  // the column scan splits the workgroups' column in this many runs
+constexpr uint32_t BIN_BRICK_THREADS = 512;
+static_assert(kSplatBinGroups <= BIN_SEGMENTS * 16u, "the column scan holds 16 counts per thread");
+static_assert(kSplatBinMaxBricks <= BIN_THREADS * 16u, "the brick scan holds 16 bricks per thread");
+
+// The voxel splat_kernel's one-corner deposit picks for particle i: the same
+// float operations (this file is built with contraction off) and the same
+// pcg4d draw, so it is the same voxel.
+GARG_DEV bool stochastic_voxel(const SplatArgs& a, uint32_t i, uint32_t voxel[3]) {
+    const float half[3] = {a.half_xy, a.half_xy, a.half_z};
+    const float pos[3] = {a.p.px[i], a.p.py[i], a.p.pz[i]};
+    float cube[3];
+    bool outside = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        cube[k] = (pos[k] + half[k]) / (2.0f * half[k]);
+        outside = outside || cube[k] < 0.0f || cube[k] >= 1.0f;
+    }
+    if (outside) return false;
+    const int n = (int)a.size;
+    int low[3], high[3];
+    float fraction[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float grid = cube[k] * (float)a.size;
+        grid = grid - 0.5f;
+        float base = floorf(grid);
+        fraction[k] = grid - base;
+        low[k] = (int)base;
+        high[k] = min(low[k] + 1, n - 1);
+        low[k] = max(low[k], 0);
+    }
+    const uint32_t frame = a.frame_counter ? *a.frame_counter : a.frame;
+    uint4 h = pcg4d(make_uint4(i, frame, a.seed_lo, a.seed_hi));
+    voxel[0] = (uint32_t)(unit_float(h.x) < fraction[0] ? high[0] : low[0]);
+    voxel[1] = (uint32_t)(unit_float(h.y) < fraction[1] ? high[1] : low[1]);
+    voxel[2] = (uint32_t)(unit_float(h.z) < fraction[2] ? high[2] : low[2]);
+    return true;
+}
+
+}  // namespace
+
+// Count pass: workgroup w takes particles [w * chunk, (w + 1) * chunk), keys
+// each by brick and voxel within the brick (~0 outside the cube) and writes
+// its brick histogram as row w of the counts.
+__global__ __launch_bounds__(BIN_THREADS) void splat_bin_count_kernel(SplatArgs a, SplatBins g) {
+    __shared__ uint32_t hist[kSplatBinMaxBricks];
+    for (uint32_t b = threadIdx.x; b < g.bricks; b += BIN_THREADS) hist[b] = 0u;
+    __syncthreads();
+    const uint32_t begin = blockIdx.x * g.chunk;
+    const uint32_t end = min(begin + g.chunk, a.p.count);
+    for (uint32_t i = begin + threadIdx.x; i < end; i += BIN_THREADS) {
+        uint32_t v[3], key = 0xffffffffu;
+        if (stochastic_voxel(a, i, v)) {
+            const uint32_t brick = ((v[2] / BIN_Z) * g.nby + v[1] / BIN_Y) * g.nbx + v[0] / BIN_X;
+            key = brick * BIN_VOXELS + ((v[2] % BIN_Z) * BIN_Y + v[1] % BIN_Y) * BIN_X + v[0] % BIN_X;
+            atomicAdd(&hist[brick], 1u);
+        }
+        g.keys[i] = key;
+    }
+    __syncthreads();
+    uint32_t* row = g.counts + (size_t)blockIdx.x * g.bricks;
+    for (uint32_t b = threadIdx.x; b < g.bricks; b += BIN_THREADS) row[b] = hist[b];
+}
+
+// Column scan: for 64 bricks per workgroup, each brick's counts down the
+// workgroups become exclusive offsets (in place), in BIN_SEGMENTS runs of at
+// most 16 workgroups summed in parallel; the brick's total goes to totals.
+__global__ __launch_bounds__(BIN_THREADS) void splat_bin_scan_kernel(SplatBins g) {
+    __shared__ uint32_t run_sum[BIN_SEGMENTS][64];
+    const uint32_t lane = threadIdx.x % 64u, run = threadIdx.x / 64u;
+    const uint32_t b = blockIdx.x * 64u + lane;
+    const uint32_t per = (g.groups + BIN_SEGMENTS - 1u) / BIN_SEGMENTS;
+    const uint32_t w0 = run * per;
+    uint32_t v[16], sum = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 16u; ++j) {
+        const uint32_t w = w0 + j;
+        v[j] = (b < g.bricks && j < per && w < g.groups) ? g.counts[(size_t)w * g.bricks + b] : 0u;
+        sum += v[j];
+    }
+    run_sum[run][lane] = sum;
+    __syncthreads();
+    if (b >= g.bricks) return;
+    uint32_t offset = 0u;
+    for (uint32_t r = 0; r < run; ++r) offset += run_sum[r][lane];
+#pragma unroll
+    for (uint32_t j = 0; j < 16u; ++j) {
+        const uint32_t w = w0 + j;
+        if (j < per && w < g.groups) {
+            g.counts[(size_t)w * g.bricks + b] = offset;
+            offset += v[j];
+        }
+    }
+    if (run == BIN_SEGMENTS - 1u) g.totals[b] = offset;
+}
+
+// Scatter pass: the bricks' starts are the exclusive scan of their totals
+// (every workgroup scans them; the first writes them out), each brick's
+// cursor starts at its start plus this workgroup's offset, and each particle
+// of the chunk takes its brick's next slot for its record: the voxel within
+// the brick and splat_kernel's six integers at weight one.
+__global__ __launch_bounds__(BIN_THREADS) void splat_bin_scatter_kernel(SplatArgs a, SplatBins g) {
+    __shared__ uint32_t cursor[kSplatBinMaxBricks];
+    const uint32_t per = (g.bricks + BIN_THREADS - 1u) / BIN_THREADS;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t mine = 0u;
+    for (uint32_t j = 0; j < per; ++j)
+        if (b0 + j < g.bricks) mine += g.totals[b0 + j];
+    // Exclusive scan of `mine` over the workgroup: within each wave by
+    // shuffles, then over the waves' totals (held in cursor[0..15] for the
+    // moment).
+    const uint32_t lane = threadIdx.x % 64u, wave = threadIdx.x / 64u;
+    uint32_t inclusive = mine;
+#pragma unroll
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t up = __shfl_up(inclusive, d, 64);
+        if (lane >= d) inclusive += up;
+    }
+    if (lane == 63u) cursor[wave] = inclusive;
+    __syncthreads();
+    uint32_t start = inclusive - mine;
+    for (uint32_t w = 0; w < wave; ++w) start += cursor[w];
+    __syncthreads();
+    const uint32_t* offsets = g.counts + (size_t)blockIdx.x * g.bricks;
+    for (uint32_t j = 0; j < per; ++j) {
+        const uint32_t b = b0 + j;
+        if (b >= g.bricks) break;
+        if (blockIdx.x == 0) g.starts[b] = start;
+        cursor[b] = start + offsets[b];
+        start += g.totals[b];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == BIN_THREADS - 1u) g.starts[g.bricks] = start;
+    __syncthreads();
+    const uint32_t begin = blockIdx.x * g.chunk;
+    const uint32_t end = min(begin + g.chunk, a.p.count);
+    for (uint32_t i = begin + threadIdx.x; i < end; i += BIN_THREADS) {
+        const uint32_t key = g.keys[i];
+        if (key == 0xffffffffu) continue;
+        const uint32_t slot = atomicAdd(&cursor[key / BIN_VOXELS], 1u);
+        float relative = a.p.temperature[i] / a.peak_temperature;
+        float energy = relative * relative;
+        energy = energy * relative;
+        energy = energy * relative;
+        energy = energy * a.emission_scale;
+        const Contribution c = contribution(1.0f, energy, relative, a.p.vx[i], a.p.vy[i], a.p.vz[i]);
+        g.records[2u * slot + 0u] = make_uint4(key % BIN_VOXELS, c.e, c.et, (uint32_t)c.mx);
+        g.records[2u * slot + 1u] = make_uint4((uint32_t)c.my, (uint32_t)c.mz, c.mass, 0u);
+    }
+}
+
+// One workgroup per brick: its records are one contiguous run; their
+// integers go into an LDS box with LDS atomics, and the box is added to the
+// brick's sums (this workgroup is their only writer in the launch).
+__global__ __launch_bounds__(BIN_BRICK_THREADS) void splat_bin_brick_kernel(SplatArgs a, SplatBins g) {
+    __shared__ uint32_t box[6 * BIN_VOXELS];
+    const uint32_t brick = blockIdx.x;
+    const uint32_t s0 = g.starts[brick], s1 = g.starts[brick + 1u];
+    if (s0 == s1) return;
+    for (uint32_t w = threadIdx.x; w < 6 * BIN_VOXELS; w += BIN_BRICK_THREADS) box[w] = 0u;
+    __syncthreads();
+    for (uint32_t s = s0 + threadIdx.x; s < s1; s += BIN_BRICK_THREADS) {
+        const uint4 r0 = g.records[2u * s + 0u], r1 = g.records[2u * s + 1u];
+        const uint32_t at = r0.x;
+        if (r0.y) atomicAdd(&box[at], r0.y);
+        if (r0.z) atomicAdd(&box[BIN_VOXELS + at], r0.z);
+        if (r0.w) atomicAdd(&box[2 * BIN_VOXELS + at], r0.w);
+        if (r1.x) atomicAdd(&box[3 * BIN_VOXELS + at], r1.x);
+        if (r1.y) atomicAdd(&box[4 * BIN_VOXELS + at], r1.y);
+        if (r1.z) atomicAdd(&box[5 * BIN_VOXELS + at], r1.z);
+    }
+    __syncthreads();
+    const uint32_t n = a.size;
+    const uint32_t x0 = (brick % g.nbx) * BIN_X, y0 = ((brick / g.nbx) % g.nby) * BIN_Y;
+    const uint32_t z0 = (brick / (g.nbx * g.nby)) * BIN_Z;
+    for (uint32_t at = threadIdx.x; at < BIN_VOXELS; at += BIN_BRICK_THREADS) {
+        const uint32_t x = x0 + at % BIN_X, y = y0 + (at / BIN_X) % BIN_Y, z = z0 + at / (BIN_X * BIN_Y);
+        if (x >= n || y >= n || z >= n) continue;
+        const uint32_t e = box[at], et = box[BIN_VOXELS + at], mx = box[2 * BIN_VOXELS + at];
+        const uint32_t my = box[3 * BIN_VOXELS + at], mz = box[4 * BIN_VOXELS + at], mass = box[5 * BIN_VOXELS + at];
+        if ((e | et | mx | my | mz | mass) == 0u) continue;
+        const uint32_t index = (z * n + y) * n + x;
+        uint32_t* em = a.emission + 2u * index;
+        uint32_t* mo = reinterpret_cast<uint32_t*>(a.momentum) + 3u * index;
+        em[0] += e;
+        em[1] += et;
+        mo[0] += mx;
+        mo[1] += my;
+        mo[2] += mz;
+        a.mass[index] += mass;
+    }
+}
+
+SplatBins splat_bin_layout(uint32_t size) {
+    SplatBins g{};
+    g.nbx = (size + BIN_X - 1u) / BIN_X;
+    g.nby = (size + BIN_Y - 1u) / BIN_Y;
+    const uint64_t bricks = (uint64_t)g.nbx * g.nby * ((size + BIN_Z - 1u) / BIN_Z);
+    g.bricks = bricks <= kSplatBinMaxBricks ? (uint32_t)bricks : 0u;
+    return g;
+}
+
+uint64_t splat_bin_table_words(uint32_t size) {
+    const SplatBins g = splat_bin_layout(size);
+    return g.bricks ? (uint64_t)kSplatBinGroups * g.bricks + 2u * g.bricks + 1u : 0u;
+}
+
+hipError_t launch_splat_binned(const SplatArgs& a, SplatBins g, hipStream_t stream) {
+    const uint32_t count = a.p.count;
+    if (g.bricks == 0 || count == 0) return hipErrorInvalidValue;
+    g.groups = std::max(1u, std::min(kSplatBinGroups, (count + 4095u) / 4096u));
+    g.chunk = (uint32_t)(((uint64_t)count + g.groups - 1u) / g.groups);
+    splat_bin_count_kernel<<<dim3(g.groups), BIN_THREADS, 0, stream>>>(a, g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    splat_bin_scan_kernel<<<dim3((g.bricks + 63u) / 64u), BIN_THREADS, 0, stream>>>(g);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    splat_bin_scatter_kernel<<<dim3(g.groups), BIN_THREADS, 0, stream>>>(a, g);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    splat_bin_brick_kernel<<<dim3(g.bricks), BIN_BRICK_THREADS, 0, stream>>>(a, g);
+    return hipGetLastError();
+}
+
 namespace {
 
 // Onesweep at every size above one block: rocprim's default picks a merge

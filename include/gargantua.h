@@ -242,6 +242,20 @@ int garg_splat_bin(garg_volume* volume, garg_particles* particles, garg_stream* 
  * the same sums, bit for bit (integers added modulo 2^32). */
 int garg_volume_set_splat_brick(garg_volume* volume, uint32_t x, uint32_t y, uint32_t z);
 int garg_volume_splat_brick(garg_volume* volume, uint32_t shape[3]);
+/* No counterpart in the reference (its deposit is one pass of device-scope
+ * atomics, splat.metal:25-88): how garg_splat_bin deposits into this volume.
+ * GARG_SPLAT_SORTED (the default) sorts instead of adding to random words:
+ * the exact deposit by cell with the library radix sort (outside graph
+ * captures), the one-corner deposit by 16x16x8 voxel brick with a counting
+ * sort of the library's own kernels (captured or not; volume sides up to
+ * 320, beyond which it deposits directly). GARG_SPLAT_ATOMIC always runs the
+ * reference's atomic pass. Both give the same sums, bit for bit. The
+ * one-corner sort uses the particle store's sort buffers, like the exact one:
+ * deposits that share a particle store must not run concurrently. */
+#define GARG_SPLAT_SORTED 0u
+#define GARG_SPLAT_ATOMIC 1u
+int garg_volume_set_splat_method(garg_volume* volume, uint32_t method);
+int garg_volume_splat_method(garg_volume* volume, uint32_t* method);
 /* resolveVolume (splat.metal:93-120); clear_sums = 1 also zeroes the sums. */
 int garg_splat_resolve(garg_volume* volume, garg_stream* stream, uint32_t clear_sums);
 /* Synchronous reads: sums (2 u32, 3 i32, 1 u32 per voxel, z-major) and the

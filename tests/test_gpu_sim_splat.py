@@ -305,3 +305,62 @@ def test_every_lds_brick_shape_gives_the_same_sums(ctx, size):
         v.brick = (4, 4, 4)
     for o in (g, s, ref, v, p):
         o.close()
+
+
+@pytest.mark.parametrize("size,hz,count", [(12, 30.0, 7), (64, 30.0, 4097), (96, 30.0, 250_000),
+                                           (128, 3.0, 1_200_000), (256, 3.0, 300_000), (320, 3.0, 20_000),
+                                           (336, 3.0, 20_000)])
+def test_binned_one_corner_deposit_equals_atomic_deposit(ctx, size, hz, count):
+    """The one-corner deposit sorted by 16x16x8 brick (the default method)
+    against the reference's pass of atomics: the same integer sums, bit for
+    bit, added onto an earlier deposit. The counts run from one count
+    workgroup (7, 4097: a partial second) to all 256 (1.2M); the sides leave
+    partial bricks (12, 320, the largest the brick tables hold) or exceed the
+    tables (336, which deposits with atomics); a 3-unit half height puts
+    most of the disk outside the cube."""
+    p = settled(ctx, count, 9, 30)
+    first = settled(ctx, 5000, 4, 5)
+    binned, atomic = (api.Volume(ctx, size, 30.0, hz) for _ in range(2))
+    assert binned.method == api.SPLAT_SORTED
+    atomic.method = api.SPLAT_ATOMIC
+    assert atomic.method == api.SPLAT_ATOMIC
+    for v in (binned, atomic):
+        v.splat(first, exact=False, frame=3)
+        v.splat(p, exact=False, frame=11)
+    got, want = binned.read_sums(), atomic.read_sums()
+    assert want[2].sum() > 0
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    with pytest.raises(api.GargError, match="unknown splat method"):
+        binned.method = 2
+    for o in (binned, atomic, p, first):
+        o.close()
+
+
+def test_captured_binned_deposit_replays_like_the_atomic_deposit(ctx):
+    """The binned one-corner deposit captured once with its frame from the
+    dispatch count and replayed after each sim dispatch, against the atomic
+    pass run eagerly with that frame: the sums agree after every replay
+    (both volumes keep adding), with the counting sort's tables rewritten by
+    each replay and no memset in the graph."""
+    p = settled(ctx, 400_000, 8, 25)
+    graphed, atomic = (api.Volume(ctx, 128, 30.0, 3.0) for _ in range(2))
+    atomic.method = api.SPLAT_ATOMIC
+    s = ctx.stream()
+    ctx.sync()
+    ctx.graph_begin(s)
+    try:
+        graphed.splat(p, exact=False, frame=api.FRAME_AUTO, stream=s)
+    finally:
+        g = ctx.graph_end(s)
+    for _ in range(3):
+        p.step(1)
+        ctx.sync()
+        g.launch(s)
+        s.sync()
+        atomic.splat(p, exact=False, frame=p.dispatches())
+        ctx.sync()
+        got, want = graphed.read_sums(), atomic.read_sums()
+        assert want[2].sum() > 0
+        assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    for o in (g, s, graphed, atomic, p):
+        o.close()
