@@ -345,16 +345,17 @@ constexpr uint32_t BIN_X = kSplatBinBrick[0], BIN_Y = kSplatBinBrick[1], BIN_Z =
 constexpr uint32_t BIN_VOXELS = BIN_X * BIN_Y * BIN_Z;
 constexpr uint32_t BIN_THREADS = 1024;
 constexpr uint32_t BIN_SEGMENTS = 16;  // the column scan splits the workgroups' column in this many runs
+constexpr uint32_t SCAN_BRICKS = BIN_THREADS / BIN_SEGMENTS;  // bricks per column-scan workgroup
 constexpr uint32_t BIN_BRICK_THREADS = 512;
+constexpr uint32_t BIN_BATCH = 8;  // particles a thread loads at once in the count and scatter passes
 static_assert(kSplatBinGroups <= BIN_SEGMENTS * 16u, "the column scan holds 16 counts per thread");
 static_assert(kSplatBinMaxBricks <= BIN_THREADS * 16u, "the brick scan holds 16 bricks per thread");
 
-// The voxel splat_kernel's one-corner deposit picks for particle i: the same
+// The voxel splat_kernel's one-corner deposit picks for particle i at `pos`: the same
 // float operations (this file is built with contraction off) and the same
 // pcg4d draw, so it is the same voxel.
-GARG_DEV bool stochastic_voxel(const SplatArgs& a, uint32_t i, uint32_t voxel[3]) {
+GARG_DEV bool stochastic_voxel(const SplatArgs& a, uint32_t i, const float pos[3], uint32_t voxel[3]) {
     const float half[3] = {a.half_xy, a.half_xy, a.half_z};
-    const float pos[3] = {a.p.px[i], a.p.py[i], a.p.pz[i]};
     float cube[3];
     bool outside = false;
 #pragma unroll
@@ -388,34 +389,51 @@ GARG_DEV bool stochastic_voxel(const SplatArgs& a, uint32_t i, uint32_t voxel[3]
 
 // Count pass: workgroup w takes particles [w * chunk, (w + 1) * chunk), keys
 // each by brick and voxel within the brick (~0 outside the cube) and writes
-// its brick histogram as row w of the counts.
+// its brick histogram as row w of the counts. A thread loads the positions
+// of BIN_BATCH particles before it works on any, so their latencies overlap.
 __global__ __launch_bounds__(BIN_THREADS) void splat_bin_count_kernel(SplatArgs a, SplatBins g) {
     __shared__ uint32_t hist[kSplatBinMaxBricks];
     for (uint32_t b = threadIdx.x; b < g.bricks; b += BIN_THREADS) hist[b] = 0u;
     __syncthreads();
     const uint32_t begin = blockIdx.x * g.chunk;
     const uint32_t end = min(begin + g.chunk, a.p.count);
-    for (uint32_t i = begin + threadIdx.x; i < end; i += BIN_THREADS) {
-        uint32_t v[3], key = 0xffffffffu;
-        if (stochastic_voxel(a, i, v)) {
-            const uint32_t brick = ((v[2] / BIN_Z) * g.nby + v[1] / BIN_Y) * g.nbx + v[0] / BIN_X;
-            key = brick * BIN_VOXELS + ((v[2] % BIN_Z) * BIN_Y + v[1] % BIN_Y) * BIN_X + v[0] % BIN_X;
-            atomicAdd(&hist[brick], 1u);
+    for (uint32_t i0 = begin + threadIdx.x; i0 < end; i0 += BIN_THREADS * BIN_BATCH) {
+        float pos[BIN_BATCH][3];
+#pragma unroll
+        for (uint32_t j = 0; j < BIN_BATCH; ++j) {
+            const uint32_t i = i0 + j * BIN_THREADS;
+            if (i < end) {
+                pos[j][0] = a.p.px[i];
+                pos[j][1] = a.p.py[i];
+                pos[j][2] = a.p.pz[i];
+            }
         }
-        g.keys[i] = key;
+#pragma unroll
+        for (uint32_t j = 0; j < BIN_BATCH; ++j) {
+            const uint32_t i = i0 + j * BIN_THREADS;
+            if (i >= end) break;
+            uint32_t v[3], key = 0xffffffffu;
+            if (stochastic_voxel(a, i, pos[j], v)) {
+                const uint32_t brick = ((v[2] / BIN_Z) * g.nby + v[1] / BIN_Y) * g.nbx + v[0] / BIN_X;
+                key = brick * BIN_VOXELS + ((v[2] % BIN_Z) * BIN_Y + v[1] % BIN_Y) * BIN_X + v[0] % BIN_X;
+                atomicAdd(&hist[brick], 1u);
+            }
+            g.keys[i] = key;
+        }
     }
     __syncthreads();
     uint32_t* row = g.counts + (size_t)blockIdx.x * g.bricks;
     for (uint32_t b = threadIdx.x; b < g.bricks; b += BIN_THREADS) row[b] = hist[b];
 }
 
-// Column scan: for 64 bricks per workgroup, each brick's counts down the
-// workgroups become exclusive offsets (in place), in BIN_SEGMENTS runs of at
-// most 16 workgroups summed in parallel; the brick's total goes to totals.
+// Column scan: for SCAN_BRICKS bricks per workgroup, each brick's counts
+// down the workgroups become exclusive offsets (in place), in BIN_SEGMENTS
+// runs of at most 16 workgroups summed in parallel; the brick's total goes
+// to totals.
 __global__ __launch_bounds__(BIN_THREADS) void splat_bin_scan_kernel(SplatBins g) {
-    __shared__ uint32_t run_sum[BIN_SEGMENTS][64];
-    const uint32_t lane = threadIdx.x % 64u, run = threadIdx.x / 64u;
-    const uint32_t b = blockIdx.x * 64u + lane;
+    __shared__ uint32_t run_sum[BIN_SEGMENTS][SCAN_BRICKS];
+    const uint32_t lane = threadIdx.x % SCAN_BRICKS, run = threadIdx.x / SCAN_BRICKS;
+    const uint32_t b = blockIdx.x * SCAN_BRICKS + lane;
     const uint32_t per = (g.groups + BIN_SEGMENTS - 1u) / BIN_SEGMENTS;
     const uint32_t w0 = run * per;
     uint32_t v[16], sum = 0u;
@@ -445,7 +463,8 @@ __global__ __launch_bounds__(BIN_THREADS) void splat_bin_scan_kernel(SplatBins g
 // (every workgroup scans them; the first writes them out), each brick's
 // cursor starts at its start plus this workgroup's offset, and each particle
 // of the chunk takes its brick's next slot for its record: the voxel within
-// the brick and splat_kernel's six integers at weight one.
+// the brick and splat_kernel's six integers at weight one. As in the count
+// pass, a thread loads BIN_BATCH particles before it scatters any.
 __global__ __launch_bounds__(BIN_THREADS) void splat_bin_scatter_kernel(SplatArgs a, SplatBins g) {
     __shared__ uint32_t cursor[kSplatBinMaxBricks];
     const uint32_t per = (g.bricks + BIN_THREADS - 1u) / BIN_THREADS;
@@ -480,18 +499,34 @@ __global__ __launch_bounds__(BIN_THREADS) void splat_bin_scatter_kernel(SplatArg
     __syncthreads();
     const uint32_t begin = blockIdx.x * g.chunk;
     const uint32_t end = min(begin + g.chunk, a.p.count);
-    for (uint32_t i = begin + threadIdx.x; i < end; i += BIN_THREADS) {
-        const uint32_t key = g.keys[i];
-        if (key == 0xffffffffu) continue;
-        const uint32_t slot = atomicAdd(&cursor[key / BIN_VOXELS], 1u);
-        float relative = a.p.temperature[i] / a.peak_temperature;
-        float energy = relative * relative;
-        energy = energy * relative;
-        energy = energy * relative;
-        energy = energy * a.emission_scale;
-        const Contribution c = contribution(1.0f, energy, relative, a.p.vx[i], a.p.vy[i], a.p.vz[i]);
-        g.records[2u * slot + 0u] = make_uint4(key % BIN_VOXELS, c.e, c.et, (uint32_t)c.mx);
-        g.records[2u * slot + 1u] = make_uint4((uint32_t)c.my, (uint32_t)c.mz, c.mass, 0u);
+    for (uint32_t i0 = begin + threadIdx.x; i0 < end; i0 += BIN_THREADS * BIN_BATCH) {
+        uint32_t key[BIN_BATCH];
+        float t[BIN_BATCH], vx[BIN_BATCH], vy[BIN_BATCH], vz[BIN_BATCH];
+#pragma unroll
+        for (uint32_t j = 0; j < BIN_BATCH; ++j) {
+            const uint32_t i = i0 + j * BIN_THREADS;
+            key[j] = 0xffffffffu;
+            if (i < end) {
+                key[j] = g.keys[i];
+                t[j] = a.p.temperature[i];
+                vx[j] = a.p.vx[i];
+                vy[j] = a.p.vy[i];
+                vz[j] = a.p.vz[i];
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < BIN_BATCH; ++j) {
+            if (key[j] == 0xffffffffu) continue;
+            const uint32_t slot = atomicAdd(&cursor[key[j] / BIN_VOXELS], 1u);
+            float relative = t[j] / a.peak_temperature;
+            float energy = relative * relative;
+            energy = energy * relative;
+            energy = energy * relative;
+            energy = energy * a.emission_scale;
+            const Contribution c = contribution(1.0f, energy, relative, vx[j], vy[j], vz[j]);
+            g.records[2u * slot + 0u] = make_uint4(key[j] % BIN_VOXELS, c.e, c.et, (uint32_t)c.mx);
+            g.records[2u * slot + 1u] = make_uint4((uint32_t)c.my, (uint32_t)c.mz, c.mass, 0u);
+        }
     }
 }
 
@@ -559,7 +594,7 @@ hipError_t launch_splat_binned(const SplatArgs& a, SplatBins g, hipStream_t stre
     splat_bin_count_kernel<<<dim3(g.groups), BIN_THREADS, 0, stream>>>(a, g);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    splat_bin_scan_kernel<<<dim3((g.bricks + 63u) / 64u), BIN_THREADS, 0, stream>>>(g);
+    splat_bin_scan_kernel<<<dim3((g.bricks + SCAN_BRICKS - 1u) / SCAN_BRICKS), BIN_THREADS, 0, stream>>>(g);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     splat_bin_scatter_kernel<<<dim3(g.groups), BIN_THREADS, 0, stream>>>(a, g);
