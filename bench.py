@@ -615,14 +615,13 @@ def pass_leg(ctx, api, particles, vol):
         direct = ctx.graph_end()
     direct_ms = timed(lambda: direct.launch(), 10)
     direct.close()
-    # The resolve alone: its bytes do not depend on the sums' values, and a
-    # difference of two timings can fall below its 20 us. Then one deposit
-    # and resolve leave the volume as the other legs expect it.
-    resolve_ms = timed(lambda: vol.resolve(clear=True), 20)
-    vol.splat(particles, exact=True)
+    # The resolve after each of its own deposits (it clears only the sums a
+    # deposit reached), timed alone; it leaves the volume as the other legs
+    # expect it.
     vol.resolve(clear=True)
+    resolve_ms, touched = resolve_after(ctx, vol, lambda: vol.splat(particles, exact=True), 20)
     sim_bytes = 64 * n
-    resolve_bytes = (24 + 16 + 24) * voxels
+    resolve_bytes = (24 + 16) * voxels + 24 * touched
     # The ceiling for the bin: the same count of no-return atomics (8 corners
     # x 6 sums per particle) to random words of the sums' footprint.
     ceiling = ctx.atomic_rate(6 * voxels, 48, n)
@@ -640,9 +639,32 @@ def pass_leg(ctx, api, particles, vol):
                                 "vs_direct_atomic_ceiling": 8 * 6 * n / (bin_ms * 1e-3) / ceiling,
                                 "direct_path_ms": direct_ms,
                                 "direct_path_corner_sums_per_s": 8 * 6 * n / (direct_ms * 1e-3)},
-            "resolve": {"ms": resolve_ms, "voxels_per_s": voxels / (resolve_ms * 1e-3), "bytes": resolve_bytes,
+            "resolve": {"ms": resolve_ms, "voxels_per_s": voxels / (resolve_ms * 1e-3), "voxels_reached": touched,
+                        "bytes": resolve_bytes,
                         "GB_per_s": resolve_bytes / (resolve_ms * 1e-3) / 1e9,
                         "hbm_frac": resolve_bytes / (resolve_ms * 1e-3) / 8e12}}
+
+
+def resolve_after(ctx, vol, deposit, reps):
+    """The resolve with its fused clear, each time after a deposit of its
+    own (the clear writes only the voxels a deposit reached), timed alone
+    with events around it; also the number of voxels one deposit reaches."""
+    deposit()
+    sums = vol.read_sums()
+    n = len(sums[2])
+    reached = int(np.count_nonzero((sums[0].reshape(n, 2) != 0).any(1) | (sums[1].reshape(n, 3) != 0).any(1)
+                                   | (sums[2] != 0)))
+    vol.resolve(clear=True)
+    total = 0.0
+    for _ in range(reps):
+        deposit()
+        a, b = ctx.event(), ctx.event()
+        a.record()
+        vol.resolve(clear=True)
+        b.record()
+        b.sync()
+        total += a.elapsed_ms(b)
+    return total / reps, reached
 
 
 def max_preset_passes(ctx, api, timed):
@@ -670,7 +692,9 @@ def max_preset_passes(ctx, api, timed):
     v.method = api.SPLAT_ATOMIC
     atomic_ms = timed(stochastic, 20)
     v.method = api.SPLAT_SORTED
-    resolve_ms = timed(lambda: v.resolve(clear=True), 20)
+    v.resolve(clear=True)
+    resolve_ms, touched = resolve_after(ctx, v, stochastic, 20)
+    resolve_bytes = 40 * voxels + 24 * touched
     ceiling = ctx.atomic_rate(6 * voxels, 6, n)
     out = {"sim": {"ms": sim_ms, "bytes": 64 * n, "GB_per_s": 64 * n / (sim_ms * 1e-3) / 1e9,
                    "hbm_frac": 64 * n / (sim_ms * 1e-3) / 8e12},
@@ -684,8 +708,9 @@ def max_preset_passes(ctx, api, timed):
                                 "atomic_pass": {"ms": atomic_ms, "atomics_per_s": 6 * n / (atomic_ms * 1e-3),
                                                 "atomic_ceiling_per_s": ceiling,
                                                 "vs_atomic_ceiling": 6 * n / (atomic_ms * 1e-3) / ceiling}},
-           "resolve": {"ms": resolve_ms, "bytes": 64 * voxels, "GB_per_s": 64 * voxels / (resolve_ms * 1e-3) / 1e9,
-                       "hbm_frac": 64 * voxels / (resolve_ms * 1e-3) / 8e12}}
+           "resolve": {"ms": resolve_ms, "voxels_reached": touched, "bytes": resolve_bytes,
+                       "GB_per_s": resolve_bytes / (resolve_ms * 1e-3) / 1e9,
+                       "hbm_frac": resolve_bytes / (resolve_ms * 1e-3) / 8e12}}
     v.close()
     p.close()
     return out

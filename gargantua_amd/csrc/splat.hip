@@ -127,7 +127,9 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveArgs a) {
     out.emission = h4{(_Float16)e.x, (_Float16)e.y, (_Float16)e.z, (_Float16)density};
     out.velocity = h4{(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)1.0f};
     a.voxels[index] = out;
-    if (a.clear) {
+    // Sums already zero need no clearing: most voxels of a disk frame stay
+    // empty, so this writes 24 bytes only for the ones a deposit reached.
+    if (a.clear && (em.x | em.y | (uint32_t)mx | (uint32_t)my | (uint32_t)mz | ms) != 0u) {
         reinterpret_cast<uint2*>(a.emission)[index] = make_uint2(0u, 0u);
         a.momentum[3u * index + 0u] = 0;
         a.momentum[3u * index + 1u] = 0;
