@@ -78,6 +78,19 @@ struct Sample {
     f3 velocity;
 };
 
+// One corner's share of a trilinear sample: each of the seven channels
+// is added as fma(texel, w, sum), one rounding per channel; a half texel
+// and a float sum make one mixed-precision FMA, with no conversion.
+GARG_DEV void add_corner(Sample& s, const Voxel& t, float w) {
+    s.emission.x = fmaf((float)t.emission.x, w, s.emission.x);
+    s.emission.y = fmaf((float)t.emission.y, w, s.emission.y);
+    s.emission.z = fmaf((float)t.emission.z, w, s.emission.z);
+    s.density = fmaf((float)t.emission.w, w, s.density);
+    s.velocity.x = fmaf((float)t.velocity.x, w, s.velocity.x);
+    s.velocity.y = fmaf((float)t.velocity.y, w, s.velocity.y);
+    s.velocity.z = fmaf((float)t.velocity.z, w, s.velocity.z);
+}
+
 // Trilinear filtering on texel centres with zero outside the cube, the
 // semantics of the reference's clamp_to_zero sampler (march.metal:15-27) as
 // the oracle states them (VolumeField.swift:40-63).
@@ -99,11 +112,8 @@ GARG_DEV Sample sample_volume(const VolumeView& v, f3 p) {
         bool inside = (unsigned)x < (unsigned)v.size && (unsigned)y < (unsigned)v.size && (unsigned)z < (unsigned)v.size;
         if (inside && w != 0.0f) {
             uint32_t i = ((uint32_t)z * (uint32_t)v.size + (uint32_t)y) * (uint32_t)v.size + (uint32_t)x;
-            const h4 em = g[i].emission;
-            const h4 ve = g[i].velocity;
-            s.emission = s.emission + f3{(float)em.x, (float)em.y, (float)em.z} * w;
-            s.density += (float)em.w * w;
-            s.velocity = s.velocity + f3{(float)ve.x, (float)ve.y, (float)ve.z} * w;
+            const Voxel t{g[i].emission, g[i].velocity};
+            add_corner(s, t, w);
         }
     }
     return s;
@@ -151,13 +161,7 @@ GARG_DEV Sample combine_corners(const TexelCorners& c) {
     for (int k = 0; k < 8; ++k) {
         float w = ((k & 1) ? c.fx : 1.0f - c.fx) * (((k >> 1) & 1) ? c.fy : 1.0f - c.fy) *
                   (((k >> 2) & 1) ? c.fz : 1.0f - c.fz);
-        if (((c.keep >> k) & 1u) && w != 0.0f) {
-            const h4 em = c.v[k].emission;
-            const h4 ve = c.v[k].velocity;
-            s.emission = s.emission + f3{(float)em.x, (float)em.y, (float)em.z} * w;
-            s.density += (float)em.w * w;
-            s.velocity = s.velocity + f3{(float)ve.x, (float)ve.y, (float)ve.z} * w;
-        }
+        if (((c.keep >> k) & 1u) && w != 0.0f) add_corner(s, c.v[k], w);
     }
     return s;
 }
@@ -285,8 +289,9 @@ struct Gather {
     // One sample at `p` after a step of length `path`; `f` is the metric
     // factor for the gravitational redshift and `direction_of` yields the
     // ray's locally measured direction when Doppler beaming needs it. FAST
-    // (the fast march forms) takes the redshift factor's fast evaluation
-    // and the hardware exp2 for the opacity.
+    // (the fast march forms) takes the redshift factor's fast evaluation,
+    // the hardware exp2 for the opacity and a product for the beaming
+    // weight's division by a constant.
     template <bool FAST = false, class DirectionOf>
     GARG_DEV void sample(const VolumeView& vol, float opacity, bool redshift, f3 p, float path, float f,
                          DirectionOf direction_of) {
@@ -299,7 +304,9 @@ struct Gather {
         if (redshift) {
             float g = redshift_factor<FAST>(f, direction_of(), s.velocity);
             float gravity = FAST ? __builtin_amdgcn_sqrtf(fmaxf(f, 0.0f)) : sqrtf(fmaxf(f, 0.0f));
-            float bulk = fminf(s.density / GARG_F(BEAMING_DENSITY_FLOOR), 1.0f);
+            float bulk = fminf(FAST ? s.density * (1.0f / GARG_F(BEAMING_DENSITY_FLOOR))
+                                    : s.density / GARG_F(BEAMING_DENSITY_FLOOR),
+                               1.0f);
             float gr3 = gravity * gravity * gravity;
             g3 = gr3 + (g * g * g - gr3) * bulk;
         }
