@@ -40,7 +40,7 @@ FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak
 # HBM bytes per launch of the batched polar-form Kerr march at this frame,
 # from two rocprofv3 counter runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the
 # factors a 256 MiB device copy reads); profiles/march_kerr_1080p_a0.9_counters.md.
-TRAFFIC_BYTES_PER_LAUNCH = 414.6e6
+TRAFFIC_BYTES_PER_LAUNCH = 415.5e6
 
 
 def parse():
