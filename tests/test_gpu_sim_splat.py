@@ -364,3 +364,42 @@ def test_captured_binned_deposit_replays_like_the_atomic_deposit(ctx):
         assert all(np.array_equal(a, b) for a, b in zip(got, want))
     for o in (g, s, graphed, atomic, p):
         o.close()
+
+
+@pytest.mark.parametrize("case", ["one_point", "brick_faces", "few"])
+def test_binned_deposit_edge_cases_equal_the_atomic_pass(ctx, case):
+    """The brick-sorted one-corner deposit where its sort and LDS gather are
+    stressed, against the atomic pass bit for bit: every particle at one
+    point (their corners are one cell's eight voxels: one hot brick, LDS
+    atomics to the same words from every lane, sums that wrap modulo 2^32),
+    particles within a voxel of the 16x16x8 bricks' faces on every axis (a
+    particle's chosen voxel on either side of a face), and a few particles
+    scattered over many bricks of a 64^3 cube."""
+    size, n = 64, 60_000
+    rng = np.random.default_rng(11)
+    count = 37 if case == "few" else n
+    p = settled(ctx, count, 17, 3)
+    rec = p.read()
+    cell = 60.0 / size
+    if case == "one_point":
+        rec["position"] = np.array([0.37, -0.21, 0.05], np.float32)
+        rec["temperature"] = np.float32(30000.0)  # large sums, so the u32 words wrap
+    elif case == "brick_faces":
+        faces = np.stack([rng.integers(1, size // 16, count) * 16, rng.integers(1, size // 16, count) * 16,
+                          rng.integers(1, size // 8, count) * 8], axis=1)
+        grid = faces + rng.uniform(-0.75, 0.75, (count, 3))  # voxel-centre coordinates around a face
+        rec["position"] = ((grid + 0.5) * cell - 30.0).astype(np.float32)
+    else:
+        rec["position"] = rng.uniform(-29.5, 29.5, (count, 3)).astype(np.float32)
+    p.write(rec)
+    binned, atomic = (api.Volume(ctx, size, 30.0, 30.0) for _ in range(2))
+    atomic.method = api.SPLAT_ATOMIC
+    for v in (binned, atomic):
+        v.splat(p, exact=False, frame=29)
+    got, want = binned.read_sums(), atomic.read_sums()
+    assert want[2].sum() > 0
+    if case == "one_point":
+        assert np.count_nonzero(want[2]) <= 8
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    for o in (binned, atomic, p):
+        o.close()
